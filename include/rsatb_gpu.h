@@ -1,0 +1,183 @@
+/*
+ * rsatb_gpu.h -- C ABI of the MI355X (gfx950) stochastic Chebyshev/KPM engine.
+ *
+ * Plain C: no exceptions cross this boundary, every call returns an int
+ * status (RG_OK = 0) and rg_last_error() holds the message of the last
+ * failure on that context. One HIP stream per context; every call is
+ * synchronous on return. Host arrays are borrowed for the duration of the
+ * call only; device buffers are owned by the context.
+ *
+ * Reference seams replaced (paths relative to the RSATB source tree):
+ *   - the operator:   Chebyshev<T> ctor + set_window + apply_op/apply_form
+ *                     rsatb/calc/cheby.hpp:167-201, 228-235, 517-520, 555-591
+ *   - the matrices:   BlockSpMat<T>::mult_dense / mult_dense_adjoint, reached
+ *                     through Chebyshev::mult_by (cheby.hpp:599-606) and
+ *                     ISQRTMPreconditioner::apply/apply_adjoint
+ *                     (rsatb/calc/preconditioner.hpp:213-227)
+ *   - the samples:    SampleVectors::initialize (rsatb/calc/observables.hpp:84-97)
+ *                     and RademacherSign (rsatb/calc/rs_ldos.hpp:256-264)
+ *   - the hot loop:   kpm_dos_moments multivector (rsatb/calc/dos.hpp:124-189)
+ *   - the DOS task:   ComputeDos kpm branch (rsatb/calc/observables.hpp:311-444)
+ *                     with evaluate_kpm_density / jackson_kernel
+ *                     (rsatb/calc/kpm.hpp:18-26, 52-85, 206-241)
+ *   - interruption:   util::throw_if_interrupted (rsatb/utils/interrupt.hpp:100-107)
+ *
+ * Host vector format of every vector argument: row-major [rows][R], either
+ * real doubles or complex doubles interleaved (re, im), as the caller states.
+ * Matrices: BSR with uniform b x b blocks, int64 block-row pointers, int32
+ * block-column indices, entries row-major within each block, blocks in CSR
+ * order -- the layout rsatb's dump_operator_coo walks
+ * (rsatb/pybind_rsatb.cpp:636-668).
+ */
+#ifndef RSATB_GPU_H
+#define RSATB_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define RG_API __attribute__((visibility("default")))
+#else
+#define RG_API
+#endif
+
+typedef struct rg_context rg_context;
+
+/* status codes */
+#define RG_OK 0
+#define RG_ERR_INVALID 1   /* bad argument or inconsistent shapes */
+#define RG_ERR_HIP 2       /* a HIP runtime call failed */
+#define RG_ERR_STATE 3     /* call out of order (operator or samples missing) */
+#define RG_ERR_CANCELLED 4 /* rg_cancel() was observed between two launches */
+#define RG_ERR_NOMEM 5     /* device allocation failed */
+#define RG_ERR_UNSUPPORTED 6
+
+/* matrix slots: H, Z (applied as Z x) and Z^H (applied as Z^H x), the three
+ * factors of the sandwich Z^H H Z (cheby.hpp:563-568). Z^H is stored as its
+ * own BSR instead of being reached through a scatter kernel. */
+#define RG_SLOT_H 0
+#define RG_SLOT_Z 1
+#define RG_SLOT_ZH 2
+#define RG_SLOT_S 3 /* overlap input of rg_isqrt_patches */
+
+/* value types of an uploaded matrix */
+#define RG_F64 0
+#define RG_C128 2
+
+/* operator forms (ChebyOpForm, cheby.hpp:78-82; Solve is not provided) */
+#define RG_FORM_ORTHOGONAL 0
+#define RG_FORM_SANDWICH 1
+
+/* sample kinds */
+#define RG_SAMPLES_RADEMACHER 0     /* RademacherSign(seed,row,col)/sqrt(rows), real */
+#define RG_SAMPLES_PHASE 1          /* e^{2 pi i u(seed,row,col)}/sqrt(rows), complex */
+#define RG_SAMPLES_UPLOAD_REAL 2    /* host_ptr: [rows][R] doubles */
+#define RG_SAMPLES_UPLOAD_COMPLEX 3 /* host_ptr: [rows][R][2] doubles */
+
+/* arithmetic of the moment loop */
+#define RG_PREC_FP64 0
+#define RG_PREC_FP32 1 /* FP32 SpMM + recurrence, FP64 compensated moment dots */
+
+/* doubling: product-identity two moments per apply (dos.hpp:150-173) */
+#define RG_DOUBLING_AUTO -1 /* on: both provided forms are Hermitian (cheby.hpp:222) */
+#define RG_DOUBLING_OFF 0
+#define RG_DOUBLING_ON 1
+
+/* KPM kernels (KpmKernel, kpm.hpp:49) */
+#define RG_KERNEL_NONE 0
+#define RG_KERNEL_JACKSON 1
+#define RG_KERNEL_COSTAPER 2
+#define RG_KERNEL_VPTAPER 3
+
+/* ---- context (no reference counterpart: the device handle) ---- */
+RG_API int rg_create(int device_id, rg_context **ctx);
+RG_API void rg_destroy(rg_context *ctx);
+RG_API const char *rg_last_error(const rg_context *ctx);
+/* Runtime version of this library, for the loader's sanity check. */
+RG_API int rg_version(void);
+
+/* ---- operands ---- */
+
+/* Replaces BlockSpMat<T> assembly for one operator slot (H, Z, Z^H, S):
+ * rsatb/setup/operator_set.hpp:291-298 produces H(k)/S(k), and
+ * preconditioner.hpp:184-196 owns Z. `values` holds nnzb*b*b doubles (F64)
+ * or nnzb*b*b (re, im) pairs (C128). */
+RG_API int rg_upload_bsr(rg_context *ctx, int slot, int64_t n_block_rows, int block_dim,
+                  const int64_t *row_ptr, const int32_t *col_ind, const void *values,
+                  int dtype);
+
+/* Chebyshev ctor + set_window (cheby.hpp:167-186, 517-520): the recurrence
+ * iterates (A - center)/half_width with A = H or Z^H H Z. */
+RG_API int rg_set_operator(rg_context *ctx, int form, double center, double half_width);
+
+/* SampleVectors::initialize (observables.hpp:84-97). host_ptr is read only
+ * for the UPLOAD kinds. */
+RG_API int rg_set_samples(rg_context *ctx, int64_t n_rows, int R, int kind, uint64_t seed,
+                   const double *host_ptr);
+
+/* Copies the current sample block to the host, [rows][R] real or
+ * [rows][R][2] complex as `as_complex` asks (checks bit-exactness). */
+RG_API int rg_get_samples(rg_context *ctx, double *host_out, int as_complex);
+
+/* ---- operator application (Chebyshev::apply_op, cheby.hpp:228-235) ---- */
+
+/* y = (A x - center x)/half_width for R columns of host vectors. */
+RG_API int rg_apply_op(rg_context *ctx, int R, const double *x, double *y, int is_complex);
+
+/* y = M x for one stored slot (BlockSpMat::mult_dense, cheby.hpp:599-606). */
+RG_API int rg_bsr_mult(rg_context *ctx, int slot, int R, const double *x, double *y, int is_complex);
+
+/* ---- the hot loop (kpm_dos_moments multivector, dos.hpp:124-189) ---- */
+
+/* mu_out: [M] complex (re, im), the mean over the R sample columns.
+ * mu_cols_out: optional [M][R] complex per-column moments (NULL to skip). */
+RG_API int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
+                   double *mu_cols_out);
+
+/* ---- DOS task (ComputeDos kpm branch, observables.hpp:311-444) ---- */
+
+/* Grid and moment count ComputeDos derives from (emin, emax, dE):
+ * nt = ceil(span/dE), N = ceil(sqrt(8 ln 2) pi half_width / dE)
+ * (observables.hpp:336-350); the output has 2*nt points. */
+RG_API int rg_dos_plan(double emin, double emax, double de, int *n_points, int *n_moments);
+
+/* Runs the moment loop and the Jackson reconstruction; dos_out and
+ * omega_out hold 2*nt values (rg_dos_plan). dos = 2 rho (observables.hpp:388-393). */
+RG_API int rg_compute_dos(rg_context *ctx, double emin, double emax, double de, int precision,
+                   double *dos_out, double *omega_out);
+
+/* evaluate_kpm_density (kpm.hpp:206-241): host FP64. mu: [M] complex. */
+RG_API int rg_kpm_density(const double *mu, int M, const double *x, int64_t nx, double edge_inset,
+                   int kernel, double *rho_out);
+
+/* ---- interruption (throw_if_interrupted, interrupt.hpp:100-107) ---- */
+
+/* Sets a flag that the moment loop checks between launches; the running
+ * call then returns RG_ERR_CANCELLED. Safe from another thread. The flag is
+ * cleared when the next loop starts. */
+RG_API int rg_cancel(rg_context *ctx);
+
+/* ---- measurement (no reference counterpart; for the benchmark) ---- */
+
+/* Turns per-launch HIP event timing on (1) or off (0) and clears stats. */
+RG_API int rg_set_profiling(rg_context *ctx, int on);
+
+/* Kernel classes timed when profiling is on. */
+#define RG_KCLASS_SPMM_PLAIN 0 /* BSR SpMM, Y = M X (sandwich inner factors) */
+#define RG_KCLASS_SPMM_STEP 1  /* SpMM fused with the recurrence and dot partials */
+#define RG_KCLASS_REDUCE 2     /* dot-partial reduction */
+#define RG_KCLASS_COUNT 3
+
+/* Average launch duration (ms), launch count, and the algorithmic bytes and
+ * flops of one launch of that class as last launched. */
+RG_API int rg_kernel_stats(rg_context *ctx, int kclass, double *avg_ms, int64_t *count,
+                    double *bytes_per_launch, double *flops_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSATB_GPU_H */

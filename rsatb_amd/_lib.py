@@ -1,0 +1,254 @@
+"""ctypes binding of librsatb_gpu.so (the C ABI of include/rsatb_gpu.h).
+
+The library is the only compute path: there is no CPU fallback. Loading
+fails loudly when the shared object has not been built, and every call that
+needs a device fails loudly when there is none.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "librsatb_gpu.so"
+
+RG_OK = 0
+RG_ERR_INVALID = 1
+RG_ERR_HIP = 2
+RG_ERR_STATE = 3
+RG_ERR_CANCELLED = 4
+RG_ERR_NOMEM = 5
+RG_ERR_UNSUPPORTED = 6
+
+SLOT_H, SLOT_Z, SLOT_ZH, SLOT_S = 0, 1, 2, 3
+F64, C128 = 0, 2
+FORM_ORTHOGONAL, FORM_SANDWICH = 0, 1
+SAMPLES_RADEMACHER, SAMPLES_PHASE, SAMPLES_UPLOAD_REAL, SAMPLES_UPLOAD_COMPLEX = 0, 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+DOUBLING_AUTO, DOUBLING_OFF, DOUBLING_ON = -1, 0, 1
+KERNEL_NONE, KERNEL_JACKSON, KERNEL_COSTAPER, KERNEL_VPTAPER = 0, 1, 2, 3
+KCLASS_SPMM_PLAIN, KCLASS_SPMM_STEP, KCLASS_REDUCE = 0, 1, 2
+KCLASS_NAMES = {KCLASS_SPMM_PLAIN: "spmm_plain", KCLASS_SPMM_STEP: "spmm_step", KCLASS_REDUCE: "reduce"}
+
+# Every symbol include/rsatb_gpu.h declares, with (restype, argtypes).
+_vp, _i, _i64, _d, _u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64
+_pd = C.POINTER(C.c_double)
+_pi64 = C.POINTER(C.c_int64)
+_pi32 = C.POINTER(C.c_int32)
+SIGNATURES = {
+    "rg_version": (_i, []),
+    "rg_create": (_i, [_i, C.POINTER(_vp)]),
+    "rg_destroy": (None, [_vp]),
+    "rg_last_error": (C.c_char_p, [_vp]),
+    "rg_upload_bsr": (_i, [_vp, _i, _i64, _i, _pi64, _pi32, _vp, _i]),
+    "rg_set_operator": (_i, [_vp, _i, _d, _d]),
+    "rg_set_samples": (_i, [_vp, _i64, _i, _i, _u64, _pd]),
+    "rg_get_samples": (_i, [_vp, _pd, _i]),
+    "rg_apply_op": (_i, [_vp, _i, _pd, _pd, _i]),
+    "rg_bsr_mult": (_i, [_vp, _i, _i, _pd, _pd, _i]),
+    "rg_kpm_moments": (_i, [_vp, _i, _i, _i, _pd, _pd]),
+    "rg_dos_plan": (_i, [_d, _d, _d, C.POINTER(_i), C.POINTER(_i)]),
+    "rg_compute_dos": (_i, [_vp, _d, _d, _d, _i, _pd, _pd]),
+    "rg_kpm_density": (_i, [_pd, _i, _pd, _i64, _d, _i, _pd]),
+    "rg_cancel": (_i, [_vp]),
+    "rg_set_profiling": (_i, [_vp, _i]),
+    "rg_kernel_stats": (_i, [_vp, _i, _pd, _pi64, _pd, _pd]),
+}
+
+_lib = None
+
+
+class RsatbGpuError(RuntimeError):
+    """A failed call into the gfx950 library (maps rsatb's std::runtime_error)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Loads the library once. Raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it first (python -m rsatb_amd.build). "
+            "rsatb_amd has no CPU fallback."
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_pd)
+
+
+def _check(ctx_handle, rc: int, what: str) -> None:
+    if rc == RG_OK:
+        return
+    msg = load().rg_last_error(ctx_handle).decode(errors="replace") if ctx_handle else ""
+    if rc == RG_ERR_CANCELLED:
+        raise KeyboardInterrupt(f"{what}: {msg or 'interrupted'}")
+    raise RsatbGpuError(rc, f"{what} failed (status {rc}): {msg}")
+
+
+def dos_plan(emin: float, emax: float, de: float) -> tuple[int, int]:
+    npts, nmom = C.c_int(0), C.c_int(0)
+    rc = load().rg_dos_plan(emin, emax, de, C.byref(npts), C.byref(nmom))
+    if rc != RG_OK:
+        raise RsatbGpuError(rc, "dos needs finite emin < emax and a positive energy resolution dE")
+    return npts.value, nmom.value
+
+
+def kpm_density(mu: np.ndarray, x: np.ndarray, edge_inset: float, kernel: int = KERNEL_JACKSON) -> np.ndarray:
+    """evaluate_kpm_density (rsatb/calc/kpm.hpp:206-241), host FP64 in the library."""
+    mu = np.ascontiguousarray(np.asarray(mu, dtype=np.complex128))
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    rho = np.zeros(x.shape[0], dtype=np.float64)
+    mu_f = mu.view(np.float64)
+    rc = load().rg_kpm_density(_dptr(mu_f), int(mu.shape[0]), _dptr(x), int(x.shape[0]), float(edge_inset), int(kernel), _dptr(rho))
+    if rc != RG_OK:
+        raise RsatbGpuError(rc, "kpm_density: bad arguments")
+    return rho
+
+
+class Context:
+    """One device context: operands, samples and the moment loop on a stream."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        rc = lib.rg_create(int(device), C.byref(h))
+        if rc != RG_OK:
+            raise RsatbGpuError(rc, f"rg_create(device={device}) failed: no usable HIP device")
+        self._h = h
+        self._lib = lib
+        self.rows = None
+        self.R = None
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.rg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # operands -------------------------------------------------------
+    def upload_bsr(self, slot: int, bsr) -> None:
+        vals = np.ascontiguousarray(bsr.values)
+        if np.iscomplexobj(vals):
+            vals = vals.astype(np.complex128, copy=False)
+            dtype = C128
+        else:
+            vals = vals.astype(np.float64, copy=False)
+            dtype = F64
+        rp = np.ascontiguousarray(bsr.row_ptr, dtype=np.int64)
+        ci = np.ascontiguousarray(bsr.col_ind, dtype=np.int32)
+        rc = self._lib.rg_upload_bsr(
+            self._h, int(slot), int(bsr.n_block_rows), int(bsr.block_dim),
+            rp.ctypes.data_as(_pi64), ci.ctypes.data_as(_pi32), vals.ctypes.data_as(C.c_void_p), dtype,
+        )
+        _check(self._h, rc, "rg_upload_bsr")
+
+    def set_operator(self, form: int, center: float, half_width: float) -> None:
+        _check(self._h, self._lib.rg_set_operator(self._h, int(form), float(center), float(half_width)), "rg_set_operator")
+
+    def set_samples(self, n_rows: int, R: int, kind: int, seed: int = 20240917, data: np.ndarray | None = None) -> None:
+        ptr = None
+        if kind == SAMPLES_UPLOAD_REAL:
+            data = np.ascontiguousarray(data, dtype=np.float64).reshape(n_rows, R)
+            ptr = _dptr(data)
+        elif kind == SAMPLES_UPLOAD_COMPLEX:
+            data = np.ascontiguousarray(data, dtype=np.complex128).reshape(n_rows, R)
+            ptr = _dptr(data.view(np.float64))
+        rc = self._lib.rg_set_samples(self._h, int(n_rows), int(R), int(kind), C.c_uint64(int(seed) & (2**64 - 1)), ptr)
+        _check(self._h, rc, "rg_set_samples")
+        self.rows, self.R = int(n_rows), int(R)
+
+    def get_samples(self, as_complex: bool = True) -> np.ndarray:
+        if as_complex:
+            out = np.zeros((self.rows, self.R), dtype=np.complex128)
+            rc = self._lib.rg_get_samples(self._h, _dptr(out.view(np.float64)), 1)
+        else:
+            out = np.zeros((self.rows, self.R), dtype=np.float64)
+            rc = self._lib.rg_get_samples(self._h, _dptr(out), 0)
+        _check(self._h, rc, "rg_get_samples")
+        return out
+
+    # operator application --------------------------------------------
+    def _vec_call(self, fn, *head, x: np.ndarray):
+        x = np.asarray(x)
+        cplx = np.iscomplexobj(x)
+        if x.ndim == 1:
+            x = x[:, None]
+        R = x.shape[1]
+        if cplx:
+            xc = np.ascontiguousarray(x, dtype=np.complex128)
+            y = np.zeros_like(xc)
+            rc = fn(self._h, *head, R, _dptr(xc.view(np.float64)), _dptr(y.view(np.float64)), 1)
+        else:
+            xc = np.ascontiguousarray(x, dtype=np.float64)
+            y = np.zeros_like(xc)
+            rc = fn(self._h, *head, R, _dptr(xc), _dptr(y), 0)
+        return rc, y
+
+    def apply_op(self, x: np.ndarray) -> np.ndarray:
+        """(A x - center x)/half_width (Chebyshev::apply_op, NORMAL)."""
+        rc, y = self._vec_call(self._lib.rg_apply_op, x=x)
+        _check(self._h, rc, "rg_apply_op")
+        return y
+
+    def bsr_mult(self, slot: int, x: np.ndarray) -> np.ndarray:
+        rc, y = self._vec_call(self._lib.rg_bsr_mult, int(slot), x=x)
+        _check(self._h, rc, "rg_bsr_mult")
+        return y
+
+    # the hot loop ------------------------------------------------------
+    def kpm_moments(self, n_moments: int, precision: int = PREC_FP64, doubling: int = DOUBLING_AUTO,
+                    per_column: bool = False):
+        mu = np.zeros(n_moments, dtype=np.complex128)
+        cols = np.zeros((n_moments, self.R), dtype=np.complex128) if per_column else None
+        rc = self._lib.rg_kpm_moments(
+            self._h, int(n_moments), int(precision), int(doubling), _dptr(mu.view(np.float64)),
+            _dptr(cols.view(np.float64)) if per_column else None,
+        )
+        _check(self._h, rc, "rg_kpm_moments")
+        return (mu, cols) if per_column else mu
+
+    def compute_dos(self, emin: float, emax: float, de: float, precision: int = PREC_FP64):
+        npts, _ = dos_plan(emin, emax, de)
+        dos = np.zeros(npts)
+        omega = np.zeros(npts)
+        rc = self._lib.rg_compute_dos(self._h, float(emin), float(emax), float(de), int(precision), _dptr(dos), _dptr(omega))
+        _check(self._h, rc, "rg_compute_dos")
+        return dos, omega
+
+    def cancel(self) -> None:
+        _check(self._h, self._lib.rg_cancel(self._h), "rg_cancel")
+
+    # measurement --------------------------------------------------------
+    def set_profiling(self, on: bool) -> None:
+        _check(self._h, self._lib.rg_set_profiling(self._h, 1 if on else 0), "rg_set_profiling")
+
+    def kernel_stats(self, kclass: int) -> dict:
+        avg, cnt, by, fl = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
+        rc = self._lib.rg_kernel_stats(self._h, int(kclass), C.byref(avg), C.byref(cnt), C.byref(by), C.byref(fl))
+        _check(self._h, rc, "rg_kernel_stats")
+        return {"avg_ms": avg.value, "count": cnt.value, "bytes": by.value, "flops": fl.value}

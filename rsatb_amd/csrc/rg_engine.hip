@@ -1,0 +1,774 @@
+// Host engine and C ABI: operands, the KPM moment loop, the DOS task.
+//
+// The loop keeps rsatb's structure (rsatb/calc/dos.hpp:124-189) but never
+// leaves the device between steps: every step enqueues the operator chain
+// (1 SpMM for the orthogonal form, 3 for the sandwich, cheby.hpp:555-591)
+// with the recurrence and the dots fused into the last SpMM, then one
+// reduction that stores the step's per-column dot products into a device
+// array. The host reads that array once, after the last step, and applies
+// the doubling identities and the column mean in FP64.
+#include "../../include/rsatb_gpu.h"
+#include "rg_internal.h"
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace rg;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct Slot {
+    bool present = false;
+    int64_t nbr = 0;
+    int32_t b = 0;
+    int64_t nnzb = 0;
+    bool is_complex = false;
+    DevBuf row_ptr, col_ind, vals64, vals32;
+    bool vals32_valid = false;
+};
+
+struct KStat {
+    double total_ms = 0.0;
+    int64_t count = 0;
+    double bytes = 0.0;
+    double flops = 0.0;
+};
+
+struct PendingEvent {
+    hipEvent_t start, stop;
+    int cls;
+};
+
+} // namespace
+
+struct rg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    Slot slots[4];
+
+    bool op_set = false;
+    int form = RG_FORM_ORTHOGONAL;
+    double center = 0.0, half_width = 1.0;
+
+    bool samples_set = false;
+    int64_t n_rows = 0;
+    int R = 0;
+    bool samples_complex = false;
+    DevBuf psi; // double or double2, as samples_complex says
+
+    DevBuf vec[6]; // t0, t1, t2, tmp1, tmp2, psi copy
+    DevBuf partials, raw;
+
+    std::atomic<int> cancel{0};
+
+    bool prof = false;
+    KStat stats[RG_KCLASS_COUNT];
+    std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+namespace {
+
+int fail(rg_context *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hipfail(rg_context *ctx, hipError_t e, const char *what) {
+    return fail(ctx, e == hipErrorOutOfMemory ? RG_ERR_NOMEM : RG_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define RG_HIP(ctx, call, what)                                                                    \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return hipfail((ctx), e_, (what));                                   \
+    } while (0)
+
+hipEvent_t take_event(rg_context *ctx) {
+    if (!ctx->pool.empty()) {
+        hipEvent_t ev = ctx->pool.back();
+        ctx->pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+// Collects elapsed times of finished launches (call after a stream sync).
+void drain_events(rg_context *ctx) {
+    for (auto &pe : ctx->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pe.start, pe.stop) == hipSuccess) {
+            ctx->stats[pe.cls].total_ms += ms;
+            ctx->stats[pe.cls].count += 1;
+        }
+        ctx->pool.push_back(pe.start);
+        ctx->pool.push_back(pe.stop);
+    }
+    ctx->pending.clear();
+}
+
+bool vec_complex(const rg_context *ctx) {
+    if (ctx->samples_complex) return true;
+    for (int s = 0; s < 3; ++s)
+        if (ctx->slots[s].present && ctx->slots[s].is_complex) return true;
+    return false;
+}
+
+size_t vec_elem_bytes(VecMode vm) { return (vm.complex_vec ? 2 : 1) * (vm.fp32 ? 4 : 8); }
+
+BsrDev dev_view(const Slot &s, bool fp32) {
+    BsrDev d;
+    d.nbr = s.nbr;
+    d.b = s.b;
+    d.nnzb = s.nnzb;
+    d.row_ptr = static_cast<const int64_t *>(s.row_ptr.p);
+    d.col_ind = static_cast<const int32_t *>(s.col_ind.p);
+    d.vals = fp32 ? s.vals32.p : s.vals64.p;
+    d.is_complex = s.is_complex;
+    return d;
+}
+
+double matrix_bytes(const Slot &s, bool fp32) {
+    const double vb = (s.is_complex ? 2.0 : 1.0) * (fp32 ? 4.0 : 8.0);
+    return (double)s.nnzb * s.b * s.b * vb + 4.0 * (double)s.nnzb + 8.0 * (double)(s.nbr + 1);
+}
+
+double matrix_flops(const Slot &s, VecMode vm, int R) {
+    const double c = s.is_complex ? 4.0 : (vm.complex_vec ? 2.0 : 1.0);
+    return 2.0 * (double)s.nnzb * s.b * s.b * (double)R * c;
+}
+
+// One SpMM launch with optional profiling; partials go to ctx->partials.
+int spmm(rg_context *ctx, int slot, VecMode vm, const void *X, void *Y, int R, EpiArgs e,
+         int *grid_out) {
+    const Slot &s = ctx->slots[slot];
+    const int cls = e.mode == EPI_PLAIN ? RG_KCLASS_SPMM_PLAIN : RG_KCLASS_SPMM_STEP;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (!a || !b) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
+        RG_HIP(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
+    }
+    hipError_t he = launch_spmm(dev_view(s, vm.fp32), vm, X, Y, R, e, ctx->stream, grid_out);
+    if (he != hipSuccess) return hipfail(ctx, he, "SpMM launch");
+    if (ctx->prof) {
+        RG_HIP(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
+        ctx->pending.push_back({a, b, cls});
+        // algorithmic bytes: matrix + each distinct vector touched once
+        const double vbytes = (double)s.nbr * s.b * (double)R * (double)vec_elem_bytes(vm);
+        const void *ptrs[5] = {X, Y, e.xs, e.xp, e.psi};
+        int distinct = 0;
+        for (int i = 0; i < 5; ++i) {
+            if (!ptrs[i]) continue;
+            bool dup = false;
+            for (int j = 0; j < i; ++j) dup = dup || (ptrs[j] == ptrs[i]);
+            if (!dup) ++distinct;
+        }
+        if (e.mode == EPI_PLAIN) distinct = 2;
+        ctx->stats[cls].bytes = matrix_bytes(s, vm.fp32) + distinct * vbytes;
+        ctx->stats[cls].flops = matrix_flops(s, vm, R);
+    }
+    return RG_OK;
+}
+
+int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (!a || !b) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
+        RG_HIP(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
+    }
+    hipError_t he = launch_reduce(static_cast<const double2 *>(ctx->partials.p), grid, ndot, R,
+                                  out, ctx->stream);
+    if (he != hipSuccess) return hipfail(ctx, he, "reduce launch");
+    if (ctx->prof) {
+        RG_HIP(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
+        ctx->pending.push_back({a, b, RG_KCLASS_REDUCE});
+        ctx->stats[RG_KCLASS_REDUCE].bytes = (double)grid * ndot * R * 16.0 + ndot * R * 16.0;
+        ctx->stats[RG_KCLASS_REDUCE].flops = (double)grid * ndot * R * 2.0;
+    }
+    return RG_OK;
+}
+
+// y = chain(x) with the epilogue of the last factor (Chebyshev::apply_form,
+// cheby.hpp:555-591): Orthogonal H x; Sandwich Z^H (H (Z x)).
+int apply_chain(rg_context *ctx, VecMode vm, const void *x, void *y, int R, EpiArgs e,
+                int *grid_out) {
+    e.xs = x;
+    if (ctx->form == RG_FORM_ORTHOGONAL) return spmm(ctx, RG_SLOT_H, vm, x, y, R, e, grid_out);
+    EpiArgs plain;
+    plain.mode = EPI_PLAIN;
+    int rc = spmm(ctx, RG_SLOT_Z, vm, x, ctx->vec[3].p, R, plain, nullptr);
+    if (rc) return rc;
+    rc = spmm(ctx, RG_SLOT_H, vm, ctx->vec[3].p, ctx->vec[4].p, R, plain, nullptr);
+    if (rc) return rc;
+    return spmm(ctx, RG_SLOT_ZH, vm, ctx->vec[4].p, y, R, e, grid_out);
+}
+
+int check_operator(rg_context *ctx) {
+    if (!ctx->op_set) return fail(ctx, RG_ERR_STATE, "operator not set (rg_set_operator)");
+    const Slot &H = ctx->slots[RG_SLOT_H];
+    if (!H.present) return fail(ctx, RG_ERR_STATE, "Hamiltonian slot is empty");
+    if (ctx->form == RG_FORM_SANDWICH) {
+        const Slot &Z = ctx->slots[RG_SLOT_Z], &ZH = ctx->slots[RG_SLOT_ZH];
+        if (!Z.present || !ZH.present)
+            return fail(ctx, RG_ERR_STATE,
+                        "The sandwich form needs Z and Z^H (slots RG_SLOT_Z, RG_SLOT_ZH)");
+        if (Z.nbr != H.nbr || Z.b != H.b || ZH.nbr != H.nbr || ZH.b != H.b)
+            return fail(ctx, RG_ERR_INVALID, "Z, Z^H and H must share block rows and block size");
+    }
+    return RG_OK;
+}
+
+// FP32 copies of the matrices the form uses.
+int ensure_fp32(rg_context *ctx) {
+    const int slots[3] = {RG_SLOT_H, RG_SLOT_Z, RG_SLOT_ZH};
+    for (int i = 0; i < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++i) {
+        Slot &s = ctx->slots[slots[i]];
+        if (s.vals32_valid) continue;
+        const int64_t n = s.nnzb * s.b * s.b * (s.is_complex ? 2 : 1);
+        RG_HIP(ctx, s.vals32.ensure((size_t)n * 4), "hipMalloc fp32 values");
+        RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.vals64.p),
+                                      static_cast<float *>(s.vals32.p), n, ctx->stream),
+               "fp32 conversion");
+        s.vals32_valid = true;
+    }
+    return RG_OK;
+}
+
+// Copies the sample block into `dst` in the loop's storage.
+int stage_samples(rg_context *ctx, VecMode vm, void *dst) {
+    const int64_t n = ctx->n_rows * (int64_t)ctx->R;
+    RG_HIP(ctx,
+           launch_stage(static_cast<const double *>(ctx->psi.p), ctx->samples_complex, dst,
+                        vm.complex_vec, vm.fp32, n, ctx->stream),
+           "sample staging");
+    return RG_OK;
+}
+
+int host_vec_to_dev(rg_context *ctx, const double *x, int64_t rows, int R, int is_complex,
+                    bool want_complex, void *dst) {
+    const int64_t n = rows * (int64_t)R;
+    if (is_complex && !want_complex)
+        return fail(ctx, RG_ERR_INVALID, "complex vector given to a real-vector operator");
+    if ((bool)is_complex == want_complex) {
+        RG_HIP(ctx, hipMemcpy(dst, x, (size_t)n * (want_complex ? 16 : 8), hipMemcpyHostToDevice),
+               "vector upload");
+        return RG_OK;
+    }
+    std::vector<double> wide((size_t)n * 2, 0.0);
+    for (int64_t i = 0; i < n; ++i) wide[2 * i] = x[i];
+    RG_HIP(ctx, hipMemcpy(dst, wide.data(), wide.size() * 8, hipMemcpyHostToDevice),
+           "vector upload");
+    return RG_OK;
+}
+
+int dev_vec_to_host(rg_context *ctx, const void *src, int64_t rows, int R, bool dev_complex,
+                    int want_complex, double *y) {
+    const int64_t n = rows * (int64_t)R;
+    if (dev_complex && !want_complex)
+        return fail(ctx, RG_ERR_INVALID, "result is complex; pass is_complex = 1");
+    if (dev_complex == (bool)want_complex) {
+        RG_HIP(ctx, hipMemcpy(y, src, (size_t)n * (dev_complex ? 16 : 8), hipMemcpyDeviceToHost),
+               "vector download");
+        return RG_OK;
+    }
+    std::vector<double> tmp((size_t)n);
+    RG_HIP(ctx, hipMemcpy(tmp.data(), src, (size_t)n * 8, hipMemcpyDeviceToHost),
+           "vector download");
+    for (int64_t i = 0; i < n; ++i) {
+        y[2 * i] = tmp[i];
+        y[2 * i + 1] = 0.0;
+    }
+    return RG_OK;
+}
+
+int64_t op_rows(const rg_context *ctx) {
+    return ctx->slots[RG_SLOT_H].nbr * (int64_t)ctx->slots[RG_SLOT_H].b;
+}
+
+} // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+int rg_version(void) { return 1; }
+
+int rg_create(int device_id, rg_context **out) {
+    if (!out) return RG_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return RG_ERR_HIP;
+    if (device_id < 0 || device_id >= n) return RG_ERR_INVALID;
+    if (hipSetDevice(device_id) != hipSuccess) return RG_ERR_HIP;
+    rg_context *ctx = new rg_context();
+    ctx->device = device_id;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return RG_ERR_HIP;
+    }
+    *out = ctx;
+    return RG_OK;
+}
+
+void rg_destroy(rg_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &pe : ctx->pending) {
+        (void)hipEventDestroy(pe.start);
+        (void)hipEventDestroy(pe.stop);
+    }
+    for (auto ev : ctx->pool) (void)hipEventDestroy(ev);
+    for (auto &s : ctx->slots) {
+        s.row_ptr.release();
+        s.col_ind.release();
+        s.vals64.release();
+        s.vals32.release();
+    }
+    ctx->psi.release();
+    for (auto &v : ctx->vec) v.release();
+    ctx->partials.release();
+    ctx->raw.release();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *rg_last_error(const rg_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *row_ptr,
+                  const int32_t *col_ind, const void *values, int dtype) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (slot < 0 || slot > 3) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
+    if (nbr < 1 || b < 1 || b > 16) return fail(ctx, RG_ERR_INVALID, "block rows >= 1 and 1 <= b <= 16");
+    if (!row_ptr || !col_ind || !values) return fail(ctx, RG_ERR_INVALID, "null array");
+    if (dtype != RG_F64 && dtype != RG_C128)
+        return fail(ctx, RG_ERR_UNSUPPORTED, "values must be RG_F64 or RG_C128");
+    if (row_ptr[0] != 0) return fail(ctx, RG_ERR_INVALID, "row_ptr[0] must be 0");
+    for (int64_t i = 0; i < nbr; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(ctx, RG_ERR_INVALID, "row_ptr must be non-decreasing");
+    const int64_t nnzb = row_ptr[nbr];
+    for (int64_t s = 0; s < nnzb; ++s)
+        if (col_ind[s] < 0 || col_ind[s] >= nbr)
+            return fail(ctx, RG_ERR_INVALID, "block column index out of range (square operator)");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    Slot &s = ctx->slots[slot];
+    s.present = false;
+    s.vals32_valid = false;
+    const size_t nv = (size_t)nnzb * b * b * (dtype == RG_C128 ? 2 : 1);
+    RG_HIP(ctx, s.row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc row_ptr");
+    RG_HIP(ctx, s.col_ind.ensure((size_t)nnzb * 4 + 4), "hipMalloc col_ind");
+    RG_HIP(ctx, s.vals64.ensure(nv * 8 + 8), "hipMalloc values");
+    RG_HIP(ctx, hipMemcpy(s.row_ptr.p, row_ptr, (size_t)(nbr + 1) * 8, hipMemcpyHostToDevice), "upload row_ptr");
+    if (nnzb > 0) {
+        RG_HIP(ctx, hipMemcpy(s.col_ind.p, col_ind, (size_t)nnzb * 4, hipMemcpyHostToDevice), "upload col_ind");
+        RG_HIP(ctx, hipMemcpy(s.vals64.p, values, nv * 8, hipMemcpyHostToDevice), "upload values");
+    }
+    s.nbr = nbr;
+    s.b = b;
+    s.nnzb = nnzb;
+    s.is_complex = (dtype == RG_C128);
+    s.present = true;
+    return RG_OK;
+}
+
+int rg_set_operator(rg_context *ctx, int form, double center, double half_width) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (form != RG_FORM_ORTHOGONAL && form != RG_FORM_SANDWICH)
+        return fail(ctx, RG_ERR_UNSUPPORTED, "form must be orthogonal or sandwich");
+    if (!std::isfinite(center) || !std::isfinite(half_width) || !(half_width > 0.0))
+        return fail(ctx, RG_ERR_INVALID, "window must be finite with half_width > 0");
+    ctx->form = form;
+    ctx->center = center;
+    ctx->half_width = half_width;
+    ctx->op_set = true;
+    return RG_OK;
+}
+
+int rg_set_samples(rg_context *ctx, int64_t n_rows, int R, int kind, uint64_t seed,
+                   const double *host_ptr) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (n_rows < 1 || R < 1) return fail(ctx, RG_ERR_INVALID, "n_rows and R must be positive");
+    if (R > 256) return fail(ctx, RG_ERR_UNSUPPORTED, "at most 256 sample columns per context");
+    if (kind < RG_SAMPLES_RADEMACHER || kind > RG_SAMPLES_UPLOAD_COMPLEX)
+        return fail(ctx, RG_ERR_INVALID, "unknown sample kind");
+    if ((kind == RG_SAMPLES_UPLOAD_REAL || kind == RG_SAMPLES_UPLOAD_COMPLEX) && !host_ptr)
+        return fail(ctx, RG_ERR_INVALID, "upload kinds need host_ptr");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const bool cplx = (kind == RG_SAMPLES_PHASE || kind == RG_SAMPLES_UPLOAD_COMPLEX);
+    const int64_t n = n_rows * (int64_t)R;
+    ctx->samples_set = false;
+    RG_HIP(ctx, ctx->psi.ensure((size_t)n * (cplx ? 16 : 8)), "hipMalloc samples");
+    if (kind == RG_SAMPLES_RADEMACHER || kind == RG_SAMPLES_PHASE) {
+        const double scale = 1.0 / std::sqrt((double)n_rows);
+        RG_HIP(ctx, launch_samples(kind == RG_SAMPLES_PHASE ? 1 : 0, seed, n_rows, R, scale, cplx,
+                                   ctx->psi.p, ctx->stream),
+               "sample kernel");
+        RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    } else {
+        RG_HIP(ctx, hipMemcpy(ctx->psi.p, host_ptr, (size_t)n * (cplx ? 16 : 8), hipMemcpyHostToDevice),
+               "sample upload");
+    }
+    ctx->n_rows = n_rows;
+    ctx->R = R;
+    ctx->samples_complex = cplx;
+    ctx->samples_set = true;
+    return RG_OK;
+}
+
+int rg_get_samples(rg_context *ctx, double *host_out, int as_complex) {
+    if (!ctx || !host_out) return RG_ERR_INVALID;
+    if (!ctx->samples_set) return fail(ctx, RG_ERR_STATE, "samples not set");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    return dev_vec_to_host(ctx, ctx->psi.p, ctx->n_rows, ctx->R, ctx->samples_complex, as_complex,
+                           host_out);
+}
+
+int rg_bsr_mult(rg_context *ctx, int slot, int R, const double *x, double *y, int is_complex) {
+    if (!ctx || !x || !y) return RG_ERR_INVALID;
+    if (slot < 0 || slot > 3 || !ctx->slots[slot].present)
+        return fail(ctx, RG_ERR_STATE, "matrix slot is empty");
+    if (R < 1 || R > 256) return fail(ctx, RG_ERR_INVALID, "1 <= R <= 256");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const Slot &s = ctx->slots[slot];
+    VecMode vm;
+    vm.complex_vec = is_complex || s.is_complex;
+    const int64_t rows = s.nbr * (int64_t)s.b;
+    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
+    RG_HIP(ctx, ctx->vec[0].ensure(nb), "hipMalloc");
+    RG_HIP(ctx, ctx->vec[1].ensure(nb), "hipMalloc");
+    int rc = host_vec_to_dev(ctx, x, rows, R, is_complex, vm.complex_vec, ctx->vec[0].p);
+    if (rc) return rc;
+    EpiArgs e;
+    e.mode = EPI_PLAIN;
+    rc = spmm(ctx, slot, vm, ctx->vec[0].p, ctx->vec[1].p, R, e, nullptr);
+    if (rc) return rc;
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->prof) drain_events(ctx);
+    return dev_vec_to_host(ctx, ctx->vec[1].p, rows, R, vm.complex_vec, is_complex, y);
+}
+
+int rg_apply_op(rg_context *ctx, int R, const double *x, double *y, int is_complex) {
+    if (!ctx || !x || !y) return RG_ERR_INVALID;
+    int rc = check_operator(ctx);
+    if (rc) return rc;
+    if (R < 1 || R > 256) return fail(ctx, RG_ERR_INVALID, "1 <= R <= 256");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    // vectors are complex when the caller's are or when a factor is
+    VecMode vm;
+    vm.complex_vec = is_complex != 0;
+    for (int sl = 0; sl < 3; ++sl)
+        vm.complex_vec = vm.complex_vec || (ctx->slots[sl].present && ctx->slots[sl].is_complex);
+    const int64_t rows = op_rows(ctx);
+    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
+    for (int i : {0, 1, 3, 4}) RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc");
+    const int grid = spmm_grid(ctx->slots[RG_SLOT_H].nbr, R);
+    RG_HIP(ctx, ctx->partials.ensure((size_t)grid * 3 * R * 16), "hipMalloc partials");
+    rc = host_vec_to_dev(ctx, x, rows, R, is_complex, vm.complex_vec, ctx->vec[0].p);
+    if (rc) return rc;
+    EpiArgs e;
+    e.mode = EPI_FIRST; // y = (A x - c x)/a; its dots are discarded here
+    e.alpha = 1.0 / ctx->half_width;
+    e.beta = -ctx->center / ctx->half_width;
+    e.partials = static_cast<double2 *>(ctx->partials.p);
+    rc = apply_chain(ctx, vm, ctx->vec[0].p, ctx->vec[1].p, R, e, nullptr);
+    if (rc) return rc;
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->prof) drain_events(ctx);
+    return dev_vec_to_host(ctx, ctx->vec[1].p, rows, R, vm.complex_vec, is_complex, y);
+}
+
+int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
+                   double *mu_cols_out) {
+    if (!ctx || !mu_out) return RG_ERR_INVALID;
+    int rc = check_operator(ctx);
+    if (rc) return rc;
+    if (!ctx->samples_set) return fail(ctx, RG_ERR_STATE, "samples not set (rg_set_samples)");
+    if (M < 1) return fail(ctx, RG_ERR_INVALID, "n_moments must be >= 1");
+    if (precision != RG_PREC_FP64 && precision != RG_PREC_FP32)
+        return fail(ctx, RG_ERR_INVALID, "unknown precision");
+    const int64_t rows = op_rows(ctx);
+    if (ctx->n_rows != rows)
+        return fail(ctx, RG_ERR_INVALID,
+                    "sample rows (" + std::to_string(ctx->n_rows) + ") differ from operator rows (" +
+                        std::to_string(rows) + ")");
+    const bool doubled = (doubling != RG_DOUBLING_OFF); // both forms are Hermitian
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->cancel.store(0);
+
+    const int R = ctx->R;
+    VecMode vm;
+    vm.complex_vec = vec_complex(ctx);
+    vm.fp32 = (precision == RG_PREC_FP32);
+    if (vm.fp32) {
+        rc = ensure_fp32(ctx);
+        if (rc) return rc;
+    }
+    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
+    for (int i = 0; i < 6; ++i) {
+        if ((i == 3 || i == 4) && ctx->form != RG_FORM_SANDWICH) continue;
+        if (i == 5 && doubled) continue;
+        RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
+    }
+    const int grid = spmm_grid(ctx->slots[RG_SLOT_H].nbr, R);
+    RG_HIP(ctx, ctx->partials.ensure((size_t)grid * 3 * R * 16), "hipMalloc partials");
+    const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
+    RG_HIP(ctx, ctx->raw.ensure((size_t)nsteps * 3 * R * 16), "hipMalloc moment store");
+    double2 *raw = static_cast<double2 *>(ctx->raw.p);
+
+    void *t0 = ctx->vec[0].p, *t1 = ctx->vec[1].p, *t2 = ctx->vec[2].p;
+    rc = stage_samples(ctx, vm, t0);
+    if (rc) return rc;
+    if (!doubled) {
+        RG_HIP(ctx, hipMemcpyAsync(ctx->vec[5].p, t0, nb, hipMemcpyDeviceToDevice, ctx->stream),
+               "psi copy");
+    }
+
+    const double a = ctx->half_width, c = ctx->center;
+    EpiArgs e;
+    e.partials = static_cast<double2 *>(ctx->partials.p);
+
+    // t1 = A~ t0 with <t0|t0>, <t0|t1>, <t1|t1> (dos.hpp:141-148, 160-163 at n = 1)
+    e.mode = EPI_FIRST;
+    e.alpha = 1.0 / a;
+    e.beta = -c / a;
+    int g = 0;
+    rc = apply_chain(ctx, vm, t0, t1, R, e, &g);
+    if (rc) return rc;
+    rc = reduce(ctx, g, 3, R, raw);
+    if (rc) return rc;
+
+    int nraw = 1;
+    e.alpha = 2.0 / a;
+    e.beta = -2.0 * c / a;
+    if (doubled) {
+        // t1 = psi_n, t0 = psi_{n-1}; t2 = psi_{n+1} with <t2|t2>, <t2|t1>
+        e.mode = EPI_STEP;
+        for (int n = 1; 2 * n + 1 < M; ++n) {
+            if (ctx->cancel.load()) {
+                (void)hipStreamSynchronize(ctx->stream);
+                drain_events(ctx);
+                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+            }
+            e.xp = t0;
+            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
+            if (rc) return rc;
+            rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R);
+            if (rc) return rc;
+            ++nraw;
+            void *sw = t0;
+            t0 = t1;
+            t1 = t2;
+            t2 = sw;
+        }
+    } else {
+        e.mode = EPI_PLAINSTEP;
+        e.psi = ctx->vec[5].p;
+        for (int m = 2; m < M; ++m) {
+            if (ctx->cancel.load()) {
+                (void)hipStreamSynchronize(ctx->stream);
+                drain_events(ctx);
+                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+            }
+            e.xp = t0;
+            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
+            if (rc) return rc;
+            rc = reduce(ctx, g, 1, R, raw + (size_t)nraw * 3 * R);
+            if (rc) return rc;
+            ++nraw;
+            void *sw = t0;
+            t0 = t1;
+            t1 = t2;
+            t2 = sw;
+        }
+    }
+    std::vector<double2> host((size_t)nraw * 3 * R);
+    RG_HIP(ctx, hipMemcpyAsync(host.data(), raw, host.size() * sizeof(double2),
+                               hipMemcpyDeviceToHost, ctx->stream),
+           "moment download");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->prof) drain_events(ctx);
+
+    // per-column moments, then the column mean (dos.hpp:133-138)
+    std::vector<double> mc((size_t)M * R * 2, 0.0);
+    auto put = [&](int m, int col, double re, double im) {
+        mc[((size_t)m * R + col) * 2] = re;
+        mc[((size_t)m * R + col) * 2 + 1] = im;
+    };
+    auto rawv = [&](int step, int d, int col) { return host[((size_t)step * 3 + d) * R + col]; };
+    for (int col = 0; col < R; ++col) {
+        const double2 d0 = rawv(0, 0, col), d1 = rawv(0, 1, col), d2 = rawv(0, 2, col);
+        put(0, col, d0.x, d0.y);
+        if (M > 1) put(1, col, d1.x, d1.y);
+        if (doubled) {
+            if (M > 2) put(2, col, 2.0 * d2.x - d0.x, 2.0 * d2.y - d0.y);
+            for (int n = 1; n < nraw; ++n) {
+                const double2 e0 = rawv(n, 0, col), e1 = rawv(n, 1, col);
+                const int mo = 2 * n + 1, me = 2 * n + 2;
+                if (mo < M) put(mo, col, 2.0 * e1.x - d1.x, 2.0 * e1.y - d1.y);
+                if (me < M) put(me, col, 2.0 * e0.x - d0.x, 2.0 * e0.y - d0.y);
+            }
+        } else {
+            for (int n = 1; n < nraw; ++n) {
+                const double2 e0 = rawv(n, 0, col);
+                put(n + 1, col, e0.x, e0.y);
+            }
+        }
+    }
+    for (int m = 0; m < M; ++m) {
+        double sr = 0.0, si = 0.0;
+        for (int col = 0; col < R; ++col) {
+            sr += mc[((size_t)m * R + col) * 2];
+            si += mc[((size_t)m * R + col) * 2 + 1];
+        }
+        mu_out[2 * m] = sr / R;
+        mu_out[2 * m + 1] = si / R;
+    }
+    if (mu_cols_out) std::memcpy(mu_cols_out, mc.data(), mc.size() * sizeof(double));
+    return RG_OK;
+}
+
+int rg_cancel(rg_context *ctx) {
+    if (!ctx) return RG_ERR_INVALID;
+    ctx->cancel.store(1);
+    return RG_OK;
+}
+
+int rg_dos_plan(double emin, double emax, double de, int *n_points, int *n_moments) {
+    if (!std::isfinite(emin) || !std::isfinite(emax) || !(emin < emax)) return RG_ERR_INVALID;
+    if (!(de > 0.0)) return RG_ERR_INVALID;
+    const double span = emax - emin;
+    const double half_width = span / 2.0;
+    const int nt = std::max(1, (int)std::ceil(span / de));
+    const double jackson_fwhm = std::sqrt(8.0 * std::log(2.0)) * M_PI;
+    const int N = std::max(1, (int)std::ceil(jackson_fwhm * half_width / de));
+    if (n_points) *n_points = 2 * nt;
+    if (n_moments) *n_moments = N;
+    return RG_OK;
+}
+
+int rg_kpm_density(const double *mu, int M, const double *x, int64_t nx, double edge_inset,
+                   int kernel, double *rho) {
+    if ((M > 0 && !mu) || (nx > 0 && (!x || !rho)) || M < 0 || nx < 0) return RG_ERR_INVALID;
+    if (M == 0) {
+        for (int64_t j = 0; j < nx; ++j) rho[j] = 0.0;
+        return RG_OK;
+    }
+    std::vector<double> gk((size_t)M, 1.0);
+    const int n0 = M / 2;
+    switch (kernel) {
+    case RG_KERNEL_NONE: break;
+    case RG_KERNEL_JACKSON: {
+        const double denom = (double)(M + 1);
+        const double theta = M_PI / denom;
+        const double cot_theta = std::cos(theta) / std::sin(theta);
+        for (int i = 1; i < M; ++i)
+            gk[i] = ((denom - i) * std::cos(theta * i) + cot_theta * std::sin(theta * i)) / denom;
+        break;
+    }
+    case RG_KERNEL_COSTAPER:
+        for (int i = n0; i < M; ++i) gk[i] = 0.5 * (1.0 + std::cos(M_PI * (double)(i - n0) / (double)(M - n0)));
+        break;
+    case RG_KERNEL_VPTAPER:
+        for (int i = n0; i < M; ++i) gk[i] = (double)(M - i) / (double)(M - n0);
+        break;
+    default: return RG_ERR_INVALID;
+    }
+    for (int64_t j = 0; j < nx; ++j) {
+        const double xv = std::max(-1.0 + edge_inset, std::min(1.0 - edge_inset, x[j]));
+        double t_prev = 1.0, t_curr = xv;
+        double acc = gk[0] * mu[0];
+        if (M > 1) acc += 2.0 * gk[1] * mu[2] * t_curr;
+        for (int m = 2; m < M; ++m) {
+            const double t_next = 2.0 * xv * t_curr - t_prev;
+            acc += 2.0 * gk[m] * mu[2 * m] * t_next;
+            t_prev = t_curr;
+            t_curr = t_next;
+        }
+        rho[j] = acc / (M_PI * std::sqrt(std::max(1e-14, 1.0 - xv * xv)));
+    }
+    return RG_OK;
+}
+
+int rg_compute_dos(rg_context *ctx, double emin, double emax, double de, int precision,
+                   double *dos_out, double *omega_out) {
+    if (!ctx || !dos_out || !omega_out) return RG_ERR_INVALID;
+    int npts = 0, N = 0;
+    if (rg_dos_plan(emin, emax, de, &npts, &N) != RG_OK)
+        return fail(ctx, RG_ERR_INVALID,
+                    "dos needs finite emin < emax and a positive energy resolution dE");
+    const int nt = npts / 2;
+    const double span = emax - emin, half_width = span / 2.0, shift = (emax + emin) / 2.0;
+    std::vector<double> mu((size_t)N * 2);
+    int rc = rg_kpm_moments(ctx, N, precision, RG_DOUBLING_AUTO, mu.data(), nullptr);
+    if (rc) return rc;
+    std::vector<double> xg((size_t)npts);
+    for (int i = 0; i < nt; ++i) {
+        xg[i] = (double)(i - nt) / nt;
+        xg[nt + i] = (double)i / nt;
+    }
+    rc = rg_kpm_density(mu.data(), N, xg.data(), npts, 0.5 / nt, RG_KERNEL_JACKSON, dos_out);
+    if (rc) return fail(ctx, rc, "density reconstruction failed");
+    for (int i = 0; i < npts; ++i) dos_out[i] *= 2.0;
+    for (int i = 0; i < nt; ++i) {
+        omega_out[i] = (i - nt) * half_width / nt + shift;
+        omega_out[nt + i] = i * half_width / nt + shift;
+    }
+    return RG_OK;
+}
+
+int rg_set_profiling(rg_context *ctx, int on) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drain_events(ctx);
+    ctx->prof = (on != 0);
+    for (auto &s : ctx->stats) s = KStat{};
+    return RG_OK;
+}
+
+int rg_kernel_stats(rg_context *ctx, int kclass, double *avg_ms, int64_t *count,
+                    double *bytes_per_launch, double *flops_per_launch) {
+    if (!ctx || kclass < 0 || kclass >= RG_KCLASS_COUNT) return RG_ERR_INVALID;
+    const KStat &s = ctx->stats[kclass];
+    if (avg_ms) *avg_ms = s.count ? s.total_ms / (double)s.count : 0.0;
+    if (count) *count = s.count;
+    if (bytes_per_launch) *bytes_per_launch = s.bytes;
+    if (flops_per_launch) *flops_per_launch = s.flops;
+    return RG_OK;
+}
+
+} // extern "C"

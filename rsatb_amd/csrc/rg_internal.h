@@ -1,0 +1,75 @@
+// Internal declarations shared by the kernel and engine translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rg {
+
+// Device view of one BSR operand. Values are real (vals) or complex
+// interleaved (vals with is_complex), in double or float storage.
+struct BsrDev {
+    int64_t nbr = 0;
+    int32_t b = 0;
+    int64_t nnzb = 0;
+    const int64_t *row_ptr = nullptr;
+    const int32_t *col_ind = nullptr;
+    const void *vals = nullptr; // double / double2 / float / float2
+    bool is_complex = false;
+};
+
+// Epilogue of a SpMM launch. y = sum_blocks A x is finished as
+//   EPI_PLAIN      Y = y
+//   EPI_FIRST      Y = alpha y + beta Xs;           dots <Xs|Xs>, <Xs|Y>, <Y|Y>
+//   EPI_STEP       Y = alpha y + beta Xs - Xp;      dots <Y|Y>, <Y|Xs>
+//   EPI_PLAINSTEP  Y = alpha y + beta Xs - Xp;      dot  <Psi|Y>
+// Dots are per sample column, conjugate-linear in the left argument, and
+// leave one (re, im) FP64 partial per workgroup and column in `partials`
+// laid out [gridDim.x][ndot][R].
+enum Epi { EPI_PLAIN = 0, EPI_FIRST = 1, EPI_STEP = 2, EPI_PLAINSTEP = 3 };
+
+inline int epi_ndot(int mode) {
+    return mode == EPI_FIRST ? 3 : mode == EPI_STEP ? 2 : mode == EPI_PLAINSTEP ? 1 : 0;
+}
+
+struct EpiArgs {
+    int mode = EPI_PLAIN;
+    double alpha = 1.0, beta = 0.0;
+    const void *xs = nullptr;
+    const void *xp = nullptr;
+    const void *psi = nullptr;
+    double2 *partials = nullptr;
+};
+
+// Vector storage of the moment loop.
+struct VecMode {
+    bool complex_vec = false; // [rows][R] double2 instead of double
+    bool fp32 = false;        // float storage (precision form)
+};
+
+// Number of workgroups a SpMM launch uses for a matrix with nbr block rows
+// and R columns (the partial-buffer size depends on it).
+int spmm_grid(int64_t nbr, int R);
+
+// Y = epilogue(A X). X, Y: [nbr*b][R] in the storage `vm` describes.
+hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int R,
+                       const EpiArgs &e, hipStream_t stream, int *grid_out);
+
+// out[d*R + u] = sum_g partials[(g*ndot + d)*R + u], fixed order.
+hipError_t launch_reduce(const double2 *partials, int ngroups, int ndot, int R, double2 *out,
+                         hipStream_t stream);
+
+// Sample block: Rademacher (real or complex with zero imaginary part) or
+// random phase (complex). scale = 1/sqrt(rows) computed on the host.
+hipError_t launch_samples(int kind, uint64_t seed, int64_t rows, int R, double scale,
+                          bool complex_vec, void *out, hipStream_t stream);
+
+// Elementwise conversion double -> float over n scalars.
+hipError_t launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t stream);
+
+// Copies n FP64 samples (real, or complex interleaved) into loop storage:
+// widening real to complex (zero imaginary part) and/or narrowing to FP32.
+hipError_t launch_stage(const double *in, bool in_complex, void *out, bool out_complex,
+                        bool out_fp32, int64_t n, hipStream_t stream);
+
+} // namespace rg

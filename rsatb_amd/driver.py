@@ -1,0 +1,217 @@
+"""Python API mirroring rsatb's driver (rsatb/driver.py) for the KPM DOS path.
+
+    calc = RSATB(structure, model, SetupOptions(nsamples=16, energy_bounds=(-8.6, 8.6)))
+    dos, energy = calc.compute_dos(dE=0.05)
+
+or, from a pre-built neighbour graph (rsatb/driver.py:73-115,
+rsatb/pybind_rsatb.cpp:566-593):
+
+    calc = RSATB.from_graph(atom_index, atom_type, edge_index, edge_shift_vec,
+                            cell, pos, model, options)
+
+Names, argument meanings, defaults and error behaviour follow the reference:
+SetupOptions / PreconditionerOptions carry the same fields and defaults as
+their pybind bindings (rsatb/pybind_rsatb.cpp:229-341); compute_dos returns
+(dos, energy) with dos holding all k-points in one flat array
+(rsatb/driver.py:120-136); configuration errors raise RuntimeError eagerly
+(rsatb/calc/observables.hpp:316-324); an interrupted run raises
+KeyboardInterrupt (rsatb/pybind_rsatb.cpp:92-93). All arithmetic runs in the
+gfx950 library; this module only assembles operands and calls it.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .bsr import BSR
+from .lattice import Structure, edges_from_graph, neighbor_edges
+from .models import assemble
+
+
+@dataclass
+class PreconditionerOptions:
+    """rsatb PreconditionerOptions (rsatb/pybind_rsatb.cpp:229-261)."""
+
+    operator_form: str = "solve"
+    tol: float = 1e-7
+    sptol: float = 1e-5
+    maxiter: int = 0
+
+    def __post_init__(self):
+        if self.operator_form not in ("solve", "sandwich"):
+            raise RuntimeError(f"Unknown operator form '{self.operator_form}' (expected solve or sandwich)")
+
+
+@dataclass
+class SetupOptions:
+    """rsatb SetupOptions (rsatb/pybind_rsatb.cpp:263-341), plus the sample
+    distribution: "phase" (unit-norm random phase, the reference's draw,
+    rsatb/calc/observables.hpp:54-97) or "rademacher"."""
+
+    kpoints: list = field(default_factory=lambda: [[0.0, 0.0, 0.0]])
+    nsamples: int = 1
+    sample_seed: int = 20240917
+    compute_bounds: bool = True
+    energy_bounds: tuple | None = None
+    bounds_zero_tol: float = 1e-12
+    bounds_max_matvecs: int = 10000
+    preconditioner: PreconditionerOptions = field(default_factory=PreconditionerOptions)
+    sample_kind: str = "phase"
+    precision: str = "fp64"
+
+    def __post_init__(self):
+        if self.energy_bounds is not None:
+            b = list(self.energy_bounds)
+            if len(b) != 2:
+                raise RuntimeError("energy_bounds must contain exactly two values: [emin, emax].")
+            self.energy_bounds = (float(b[0]), float(b[1]))
+        if self.sample_kind not in ("phase", "rademacher"):
+            raise RuntimeError("sample_kind must be 'phase' or 'rademacher'")
+        if self.precision not in ("fp64", "fp32"):
+            raise RuntimeError("precision must be 'fp64' or 'fp32'")
+
+
+def make_explicit_bounds(emin: float, emax: float) -> tuple[float, float]:
+    """MakeExplicitBounds (rsatb/setup/spectral_bounds.hpp:427-432)."""
+    if not (math.isfinite(emin) and math.isfinite(emax) and emin < emax):
+        raise RuntimeError("Explicit energy bounds must be finite and satisfy emin < emax.")
+    return float(emin), float(emax)
+
+
+class RSATB:
+    """A calculation on one GPU: operators per k-point, samples, the DOS task."""
+
+    def __init__(self, structure: Structure, model, options: SetupOptions | None = None, npar: int = 1,
+                 device: int = 0, edges=None):
+        if options is None:
+            options = SetupOptions()
+        if npar != 1:
+            raise RuntimeError("npar > 1 (k-point groups across processes) is not provided on one GPU")
+        if options.nsamples < 1:
+            raise RuntimeError("nsamples must be positive")
+        self.structure = structure
+        self.model = model
+        self.options = options
+        self.edges = edges if edges is not None else neighbor_edges(structure, model.cutoff)
+        self._ctx = _lib.Context(device)
+        self._kstates = []
+        self._bounds = None
+        self._setup()
+
+    @classmethod
+    def from_graph(cls, atom_index, atom_type, edge_index, edge_shift_vec, cell, pos, model,
+                   options: SetupOptions | None = None, N_atom: int = 0, atom_offset: int = 0,
+                   N_edge: int = 0, npar: int = 1, device: int = 0) -> "RSATB":
+        """rsatb RSATB.from_graph: one process owns the whole graph here, so the
+        local counts must equal the totals (rsatb/rsatb.hpp:467-491)."""
+        atom_index = np.asarray(atom_index, dtype=np.int64).reshape(-1)
+        n_atom = atom_index.size
+        n_edge = np.asarray(edge_index).size // 2
+        if N_atom and N_atom != n_atom:
+            raise RuntimeError(f"N_atom={N_atom} but {n_atom} atoms were passed (one process owns all atoms)")
+        if N_edge and N_edge != n_edge:
+            raise RuntimeError(f"N_edge={N_edge} but {n_edge} edges were passed")
+        if atom_offset != 0:
+            raise RuntimeError("atom_offset must be 0 on a single process")
+        if not np.array_equal(np.sort(atom_index), np.arange(n_atom)):
+            raise RuntimeError("atom_index must be a permutation of 0..N_atom-1")
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        st = Structure(pos, np.asarray(cell, dtype=np.float64).reshape(3, 3), (True, True, True),
+                       np.asarray(atom_type, dtype=np.int32))
+        edges = edges_from_graph(st, edge_index, edge_shift_vec)
+        return cls(st, model, options, npar, device, edges=edges)
+
+    # ------------------------------------------------------------- setup
+    def _setup(self) -> None:
+        o = self.options
+        st, model = self.structure, self.model
+        sandwich = model.has_overlap()
+        if sandwich and o.preconditioner.operator_form != "sandwich":
+            raise RuntimeError(
+                "This model has an overlap: the GPU path iterates the sandwich form Z^H H Z only; "
+                "set PreconditionerOptions(operator_form='sandwich')."
+            )
+        for k in o.kpoints:
+            H = assemble(st, self.edges, model, "H", k)
+            S = assemble(st, self.edges, model, "S", k) if sandwich else None
+            self._kstates.append({"k": tuple(k), "H": H, "S": S, "Z": None})
+        if o.energy_bounds is not None:
+            self._bounds = make_explicit_bounds(*o.energy_bounds)
+        elif o.compute_bounds:
+            raise RuntimeError("automatic spectral bounds are not available yet: pass energy_bounds")
+        n_rows = st.n_atoms * model.norb
+        kind = _lib.SAMPLES_PHASE if o.sample_kind == "phase" else _lib.SAMPLES_RADEMACHER
+        self._ctx.set_samples(n_rows, o.nsamples, kind, o.sample_seed)
+
+    def _bind_kpoint(self, idx: int) -> None:
+        ks = self._kstates[idx]
+        self._ctx.upload_bsr(_lib.SLOT_H, ks["H"])
+        if ks["S"] is not None:
+            if ks["Z"] is None:
+                raise RuntimeError("the sandwich form needs Z = S^-1/2; set it with set_overlap_factor()")
+            self._ctx.upload_bsr(_lib.SLOT_Z, ks["Z"])
+            self._ctx.upload_bsr(_lib.SLOT_ZH, ks["Z"].conj_transpose())
+            form = _lib.FORM_SANDWICH
+        else:
+            form = _lib.FORM_ORTHOGONAL
+        emin, emax = self.energy_bounds()
+        self._ctx.set_operator(form, (emax + emin) / 2.0, (emax - emin) / 2.0)
+
+    def set_overlap_factor(self, Z: BSR, k_idx: int = 0) -> None:
+        """Installs a factor Z (applied as Z x, with Z^H as its adjoint) for a
+        k-point, as ISQRTMPreconditioner's borrowing constructor does
+        (rsatb/calc/preconditioner.hpp:194-196)."""
+        self._kstates[k_idx]["Z"] = Z
+
+    # ------------------------------------------------------------- tasks
+    def energy_bounds(self) -> tuple[float, float]:
+        if self._bounds is None:
+            raise RuntimeError(
+                "Spectral bound is not computed yet. Set it true during Setup stage for random state method support."
+            )
+        return self._bounds
+
+    def kpm_moments(self, n_moments: int, k_idx: int = 0, precision: str | None = None,
+                    doubling: bool | None = None, per_column: bool = False):
+        """kpm_dos_moments (rsatb/calc/dos.hpp:124-189) on one k-point."""
+        self._bind_kpoint(k_idx)
+        prec = _lib.PREC_FP32 if (precision or self.options.precision) == "fp32" else _lib.PREC_FP64
+        dbl = _lib.DOUBLING_AUTO if doubling is None else (_lib.DOUBLING_ON if doubling else _lib.DOUBLING_OFF)
+        return self._ctx.kpm_moments(n_moments, prec, dbl, per_column)
+
+    def compute_dos(self, dE: float, method: str = "kpm"):
+        """ComputeDos (rsatb/calc/observables.hpp:311-444): returns (dos, energy),
+        dos flat over k-points."""
+        if method != "kpm":
+            raise RuntimeError(f"dos method must be 'kpm' on the GPU path, got '{method}'")
+        if not (dE > 0.0):
+            raise RuntimeError("dos dE must be a positive energy resolution in eV")
+        emin, emax = self.energy_bounds()
+        prec = _lib.PREC_FP32 if self.options.precision == "fp32" else _lib.PREC_FP64
+        parts, omega = [], None
+        for k_idx in range(len(self._kstates)):
+            self._bind_kpoint(k_idx)
+            dos, omega = self._ctx.compute_dos(emin, emax, dE, prec)
+            parts.append(dos)
+        return np.concatenate(parts), omega
+
+    def dump_operator_coo(self, which: str, axis: int = 0, k_idx: int = 0):
+        """Global COO triplets of H or S (rsatb/pybind_rsatb.cpp:612-670)."""
+        ks = self._kstates[k_idx]
+        M = ks.get(which)
+        if which not in ("H", "S"):
+            raise RuntimeError(f"unknown operator name: {which}")
+        if M is None:
+            raise RuntimeError(f"{which} is not populated for this model")
+        return M.to_coo()
+
+    def cancel(self) -> None:
+        """Asks a running moment loop to stop at its next step boundary."""
+        self._ctx.cancel()
+
+    @property
+    def context(self) -> _lib.Context:
+        return self._ctx
