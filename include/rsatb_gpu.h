@@ -153,6 +153,36 @@ RG_API int rg_compute_dos(rg_context *ctx, double emin, double emax, double de, 
 RG_API int rg_kpm_density(const double *mu, int M, const double *x, int64_t nx, double edge_inset,
                    int kernel, double *rho_out);
 
+/* ---- operand inspection (dump_operator_coo, rsatb/pybind_rsatb.cpp:612-670) ---- */
+
+RG_API int rg_bsr_info(rg_context *ctx, int slot, int64_t *n_block_rows, int *block_dim,
+                       int64_t *nnzb, int *is_complex);
+/* Copies a slot back in the upload layout (values row-major within blocks);
+ * any output pointer may be NULL. */
+RG_API int rg_download_bsr(rg_context *ctx, int slot, int64_t *row_ptr, int32_t *col_ind,
+                           double *values);
+
+/* ---- graph-local S^{-1/2} (graph_inverse_sqrt seed, rsatb/calc/isqrtm.hpp:137-232;
+ *      ISQRTMPreconditioner ctor, rsatb/calc/preconditioner.hpp:184-192) ---- */
+
+/* From the overlap in RG_SLOT_S (real, symmetric pattern, diagonal blocks
+ * stored) builds Z in RG_SLOT_Z and Z^H in RG_SLOT_ZH: block row i of Z is
+ * block i's rows of S[C_i, C_i]^{-1/2}, C_i the stored columns of row i, by
+ * batched FP64 Newton-Schulz on the device. Converged when
+ * ||I - Z Y||_F < tol on every patch; fails (RG_ERR_INVALID) when an
+ * iteration diverges (residual > 100, isqrtm.hpp:637-643) or maxit is hit. */
+RG_API int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out,
+                            double *resid_out);
+
+/* ---- spectral window (EstimateSpectralBounds, rsatb/setup/spectral_bounds.hpp:445-489;
+ *      the extremes come from FP64 block Lanczos on the device instead of PRIMME) ---- */
+
+/* 16 independent Lanczos chains of `steps` steps from counter-based start
+ * vectors (seed); returns [min Ritz - padding, max Ritz + padding]. Needs the
+ * factors of `form` uploaded; the window set by rg_set_operator is not used. */
+RG_API int rg_estimate_bounds(rg_context *ctx, int form, int steps, uint64_t seed,
+                              double padding, double *emin, double *emax);
+
 /* ---- interruption (throw_if_interrupted, interrupt.hpp:100-107) ---- */
 
 /* Sets a flag that the moment loop checks between launches; the running

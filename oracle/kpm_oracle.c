@@ -59,6 +59,7 @@ typedef struct {
     const or_bsr *Z; /* sandwich: y = Z^H H Z x; Z^H applied by the scatter kernel */
     double center;
     double half_width;
+    const or_bsr *ZH; /* optional explicit Z^H: row-gather instead of the scatter */
 } or_op;
 
 /* ---------------------------------------------------------------- RNG --- */
@@ -211,7 +212,8 @@ int or_apply_op(const or_op *op, const double *x, int32_t R, double *y, double *
     } else if (op->form == OR_FORM_SANDWICH) {
         or_bsr_mult(op->Z, x, R, y);         /* M->apply(x, y) */
         or_bsr_mult(op->H, y, R, tmp);       /* mult_by(H, y, tmp) */
-        or_bsr_mult_adjoint(op->Z, tmp, R, y); /* M->apply_adjoint(tmp, y) */
+        if (op->ZH) or_bsr_mult(op->ZH, tmp, R, y);
+        else or_bsr_mult_adjoint(op->Z, tmp, R, y); /* M->apply_adjoint(tmp, y) */
     } else {
         return -1;
     }
@@ -335,7 +337,8 @@ int or_kpm_moments(const or_op *op, const double *psi, int32_t R, int32_t M, int
                 } else {
                     or_bsr_mult(op->Z, t1, R, t2);
                     or_bsr_mult(op->H, t2, R, tmp);
-                    or_bsr_mult_adjoint(op->Z, tmp, R, t2);
+                    if (op->ZH) or_bsr_mult(op->ZH, tmp, R, t2);
+                    else or_bsr_mult_adjoint(op->Z, tmp, R, t2);
                     memcpy(tmp, t2, sizeof(double) * 2 * (size_t)n);
                 }
                 fused_step(op, t0, t1, t2, tmp, R, dots, dots2);

@@ -47,6 +47,7 @@ class _Op(C.Structure):
         ("Z", C.POINTER(_Bsr)),
         ("center", C.c_double),
         ("half_width", C.c_double),
+        ("ZH", C.POINTER(_Bsr)),
     ]
 
 
@@ -154,12 +155,16 @@ def bsr_mult_adjoint(A, x) -> np.ndarray:
 class Operator:
     """The scaled operator (A - center)/half_width, A = H or Z^H H Z."""
 
-    def __init__(self, form: int, H, center: float, half_width: float, Z=None):
+    def __init__(self, form: int, H, center: float, half_width: float, Z=None, ZH=None):
+        """ZH (optional): an explicit Z^H applied by the row-gather kernel
+        instead of scattering Z's blocks (the faster CPU variant)."""
         self.form = form
         self._h = _Holder(H)
         self._z = _Holder(Z) if Z is not None else None
+        self._zh = _Holder(ZH) if ZH is not None else None
         self.rows = H.n_block_rows * H.block_dim
-        self.op = _Op(form, C.pointer(self._h.s), C.pointer(self._z.s) if self._z else None, center, half_width)
+        self.op = _Op(form, C.pointer(self._h.s), C.pointer(self._z.s) if self._z else None, center, half_width,
+                      C.pointer(self._zh.s) if self._zh else None)
 
     def apply(self, x) -> np.ndarray:
         xc = _cvec(x)

@@ -51,6 +51,10 @@ SIGNATURES = {
     "rg_dos_plan": (_i, [_d, _d, _d, C.POINTER(_i), C.POINTER(_i)]),
     "rg_compute_dos": (_i, [_vp, _d, _d, _d, _i, _pd, _pd]),
     "rg_kpm_density": (_i, [_pd, _i, _pd, _i64, _d, _i, _pd]),
+    "rg_bsr_info": (_i, [_vp, _i, _pi64, C.POINTER(_i), _pi64, C.POINTER(_i)]),
+    "rg_download_bsr": (_i, [_vp, _i, _pi64, _pi32, _pd]),
+    "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
+    "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_cancel": (_i, [_vp]),
     "rg_set_profiling": (_i, [_vp, _i]),
     "rg_kernel_stats": (_i, [_vp, _i, _pd, _pi64, _pd, _pd]),
@@ -166,6 +170,40 @@ class Context:
             rp.ctypes.data_as(_pi64), ci.ctypes.data_as(_pi32), vals.ctypes.data_as(C.c_void_p), dtype,
         )
         _check(self._h, rc, "rg_upload_bsr")
+
+    def bsr_info(self, slot: int) -> dict:
+        nbr, b, nnzb, cplx = C.c_int64(), C.c_int(), C.c_int64(), C.c_int()
+        rc = self._lib.rg_bsr_info(self._h, int(slot), C.byref(nbr), C.byref(b), C.byref(nnzb), C.byref(cplx))
+        _check(self._h, rc, "rg_bsr_info")
+        return {"n_block_rows": nbr.value, "block_dim": b.value, "nnzb": nnzb.value, "is_complex": bool(cplx.value)}
+
+    def download_bsr(self, slot: int):
+        """A slot's operand in the upload layout (rsatb_amd.bsr.BSR)."""
+        from .bsr import BSR
+
+        info = self.bsr_info(slot)
+        nbr, b, nnzb = info["n_block_rows"], info["block_dim"], info["nnzb"]
+        rp = np.zeros(nbr + 1, dtype=np.int64)
+        ci = np.zeros(max(nnzb, 1), dtype=np.int32)
+        vals = np.zeros((max(nnzb, 1), b, b), dtype=np.complex128 if info["is_complex"] else np.float64)
+        rc = self._lib.rg_download_bsr(self._h, int(slot), rp.ctypes.data_as(_pi64), ci.ctypes.data_as(_pi32),
+                                       _dptr(vals.view(np.float64)))
+        _check(self._h, rc, "rg_download_bsr")
+        return BSR(nbr, b, rp, ci[:nnzb], vals[:nnzb])
+
+    def isqrt_patches(self, tol: float = 1e-12, maxit: int = 60) -> tuple[int, float]:
+        """Z, Z^H from the overlap in SLOT_S (graph_inverse_sqrt seed)."""
+        it, res = C.c_int(), C.c_double()
+        rc = self._lib.rg_isqrt_patches(self._h, float(tol), int(maxit), C.byref(it), C.byref(res))
+        _check(self._h, rc, "rg_isqrt_patches")
+        return it.value, res.value
+
+    def estimate_bounds(self, form: int, steps: int = 200, seed: int = 20240917, padding: float = 0.5):
+        lo, hi = C.c_double(), C.c_double()
+        rc = self._lib.rg_estimate_bounds(self._h, int(form), int(steps), C.c_uint64(int(seed) & (2**64 - 1)),
+                                          float(padding), C.byref(lo), C.byref(hi))
+        _check(self._h, rc, "rg_estimate_bounds")
+        return lo.value, hi.value
 
     def set_operator(self, form: int, center: float, half_width: float) -> None:
         _check(self._h, self._lib.rg_set_operator(self._h, int(form), float(center), float(half_width)), "rg_set_operator")

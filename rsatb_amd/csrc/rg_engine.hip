@@ -10,9 +10,11 @@
 #include "../../include/rsatb_gpu.h"
 #include "rg_internal.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -227,9 +229,10 @@ int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out) {
 // y = chain(x) with the epilogue of the last factor (Chebyshev::apply_form,
 // cheby.hpp:555-591): Orthogonal H x; Sandwich Z^H (H (Z x)).
 int apply_chain(rg_context *ctx, VecMode vm, const void *x, void *y, int R, EpiArgs e,
-                int *grid_out) {
+                int *grid_out, int form = -1) {
+    if (form < 0) form = ctx->form;
     e.xs = x;
-    if (ctx->form == RG_FORM_ORTHOGONAL) return spmm(ctx, RG_SLOT_H, vm, x, y, R, e, grid_out);
+    if (form == RG_FORM_ORTHOGONAL) return spmm(ctx, RG_SLOT_H, vm, x, y, R, e, grid_out);
     EpiArgs plain;
     plain.mode = EPI_PLAIN;
     int rc = spmm(ctx, RG_SLOT_Z, vm, x, ctx->vec[3].p, R, plain, nullptr);
@@ -337,6 +340,9 @@ int rg_create(int device_id, rg_context **out) {
     if (hipSetDevice(device_id) != hipSuccess) return RG_ERR_HIP;
     rg_context *ctx = new rg_context();
     ctx->device = device_id;
+    if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
+    if (const char *v = std::getenv("RG_MFMA_PREFETCH")) set_mfma_prefetch(std::atoi(v));
+    if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RG_ERR_HIP;
@@ -396,7 +402,17 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     RG_HIP(ctx, hipMemcpy(s.row_ptr.p, row_ptr, (size_t)(nbr + 1) * 8, hipMemcpyHostToDevice), "upload row_ptr");
     if (nnzb > 0) {
         RG_HIP(ctx, hipMemcpy(s.col_ind.p, col_ind, (size_t)nnzb * 4, hipMemcpyHostToDevice), "upload col_ind");
-        RG_HIP(ctx, hipMemcpy(s.vals64.p, values, nv * 8, hipMemcpyHostToDevice), "upload values");
+        // stage row-major values, store column-major blocks
+        DevBuf staging;
+        RG_HIP(ctx, staging.ensure(nv * 8), "hipMalloc staging");
+        hipError_t he = hipMemcpy(staging.p, values, nv * 8, hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = launch_block_transpose(static_cast<const double *>(staging.p),
+                                        static_cast<double *>(s.vals64.p), nnzb, b,
+                                        dtype == RG_C128, ctx->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+        staging.release();
+        if (he != hipSuccess) return hipfail(ctx, he, "upload values");
     }
     s.nbr = nbr;
     s.b = b;
@@ -496,8 +512,7 @@ int rg_apply_op(rg_context *ctx, int R, const double *x, double *y, int is_compl
     const int64_t rows = op_rows(ctx);
     const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
     for (int i : {0, 1, 3, 4}) RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc");
-    const int grid = spmm_grid(ctx->slots[RG_SLOT_H].nbr, R);
-    RG_HIP(ctx, ctx->partials.ensure((size_t)grid * 3 * R * 16), "hipMalloc partials");
+    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
     rc = host_vec_to_dev(ctx, x, rows, R, is_complex, vm.complex_vec, ctx->vec[0].p);
     if (rc) return rc;
     EpiArgs e;
@@ -544,8 +559,7 @@ int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *
         if (i == 5 && doubled) continue;
         RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
     }
-    const int grid = spmm_grid(ctx->slots[RG_SLOT_H].nbr, R);
-    RG_HIP(ctx, ctx->partials.ensure((size_t)grid * 3 * R * 16), "hipMalloc partials");
+    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
     const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
     RG_HIP(ctx, ctx->raw.ensure((size_t)nsteps * 3 * R * 16), "hipMalloc moment store");
     double2 *raw = static_cast<double2 *>(ctx->raw.p);
@@ -659,6 +673,285 @@ int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *
         mu_out[2 * m + 1] = si / R;
     }
     if (mu_cols_out) std::memcpy(mu_cols_out, mc.data(), mc.size() * sizeof(double));
+    return RG_OK;
+}
+
+
+int rg_bsr_info(rg_context *ctx, int slot, int64_t *n_block_rows, int *block_dim, int64_t *nnzb,
+                int *is_complex) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (slot < 0 || slot > 3 || !ctx->slots[slot].present)
+        return fail(ctx, RG_ERR_STATE, "matrix slot is empty");
+    const Slot &s = ctx->slots[slot];
+    if (n_block_rows) *n_block_rows = s.nbr;
+    if (block_dim) *block_dim = s.b;
+    if (nnzb) *nnzb = s.nnzb;
+    if (is_complex) *is_complex = s.is_complex ? 1 : 0;
+    return RG_OK;
+}
+
+int rg_download_bsr(rg_context *ctx, int slot, int64_t *row_ptr, int32_t *col_ind, double *values) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (slot < 0 || slot > 3 || !ctx->slots[slot].present)
+        return fail(ctx, RG_ERR_STATE, "matrix slot is empty");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const Slot &s = ctx->slots[slot];
+    if (row_ptr)
+        RG_HIP(ctx, hipMemcpy(row_ptr, s.row_ptr.p, (size_t)(s.nbr + 1) * 8, hipMemcpyDeviceToHost), "download");
+    if (col_ind && s.nnzb)
+        RG_HIP(ctx, hipMemcpy(col_ind, s.col_ind.p, (size_t)s.nnzb * 4, hipMemcpyDeviceToHost), "download");
+    if (values && s.nnzb) {
+        const int comp = s.is_complex ? 2 : 1;
+        const size_t per = (size_t)s.b * s.b;
+        std::vector<double> cm((size_t)s.nnzb * per * comp);
+        RG_HIP(ctx, hipMemcpy(cm.data(), s.vals64.p, cm.size() * 8, hipMemcpyDeviceToHost), "download");
+        for (int64_t q = 0; q < s.nnzb; ++q)
+            for (int r = 0; r < s.b; ++r)
+                for (int c = 0; c < s.b; ++c)
+                    for (int k = 0; k < comp; ++k)
+                        values[((q * per) + (size_t)r * s.b + c) * comp + k] =
+                            cm[((q * per) + (size_t)c * s.b + r) * comp + k];
+    }
+    return RG_OK;
+}
+
+int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, double *resid_out) {
+    if (!ctx) return RG_ERR_INVALID;
+    Slot &S = ctx->slots[RG_SLOT_S];
+    if (!S.present) return fail(ctx, RG_ERR_STATE, "overlap slot RG_SLOT_S is empty");
+    if (S.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "complex overlaps are not supported by the patch kernel");
+    if (!(tol > 0.0) || maxit < 1) return fail(ctx, RG_ERR_INVALID, "tol > 0 and maxit >= 1 required");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const int64_t nbr = S.nbr;
+    const int b = S.b;
+    std::vector<int64_t> rp((size_t)nbr + 1);
+    std::vector<int32_t> ci((size_t)S.nnzb);
+    RG_HIP(ctx, hipMemcpy(rp.data(), S.row_ptr.p, rp.size() * 8, hipMemcpyDeviceToHost), "download pattern");
+    if (S.nnzb) RG_HIP(ctx, hipMemcpy(ci.data(), S.col_ind.p, ci.size() * 4, hipMemcpyDeviceToHost), "download pattern");
+    int64_t nbmax = 0;
+    for (int64_t i = 0; i < nbr; ++i) {
+        const int32_t *b0 = ci.data() + rp[i], *b1 = ci.data() + rp[i + 1];
+        nbmax = std::max<int64_t>(nbmax, rp[i + 1] - rp[i]);
+        if (!std::is_sorted(b0, b1) || std::adjacent_find(b0, b1) != b1)
+            return fail(ctx, RG_ERR_INVALID, "overlap block columns must be sorted and unique");
+        if (!std::binary_search(b0, b1, (int32_t)i))
+            return fail(ctx, RG_ERR_INVALID, "every overlap block row must store its diagonal block");
+        for (const int32_t *q = b0; q != b1; ++q) {
+            const int32_t *c0 = ci.data() + rp[*q], *c1 = ci.data() + rp[*q + 1];
+            if (!std::binary_search(c0, c1, (int32_t)i))
+                return fail(ctx, RG_ERR_INVALID, "overlap sparsity pattern must be symmetric");
+        }
+    }
+    const int n = (int)(nbmax * b);
+    const int ld = ((n + 63) / 64) * 64;
+    if (ld > 512)
+        return fail(ctx, RG_ERR_UNSUPPORTED,
+                    "patch dimension " + std::to_string(n) + " exceeds the 512 the kernel supports");
+    const size_t mat = (size_t)ld * ld;
+    const size_t budget = (size_t)6 << 30;
+    int64_t batch = (int64_t)std::max<size_t>(1, budget / (5 * mat * 8));
+    batch = std::min<int64_t>(batch, nbr);
+    DevBuf buf[5], cs, resid;
+    for (auto &bb : buf) RG_HIP(ctx, bb.ensure((size_t)batch * mat * 8), "hipMalloc patches");
+    RG_HIP(ctx, cs.ensure((size_t)batch * 8), "hipMalloc");
+    RG_HIP(ctx, resid.ensure((size_t)batch * 8), "hipMalloc");
+    // the factor shares S's pattern
+    Slot &Z = ctx->slots[RG_SLOT_Z];
+    Slot &ZH = ctx->slots[RG_SLOT_ZH];
+    for (Slot *t : {&Z, &ZH}) {
+        t->present = false;
+        t->vals32_valid = false;
+        RG_HIP(ctx, t->row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc");
+        RG_HIP(ctx, t->col_ind.ensure((size_t)S.nnzb * 4 + 4), "hipMalloc");
+        RG_HIP(ctx, t->vals64.ensure((size_t)S.nnzb * b * b * 8 + 8), "hipMalloc");
+        RG_HIP(ctx, hipMemcpy(t->row_ptr.p, S.row_ptr.p, (size_t)(nbr + 1) * 8, hipMemcpyDeviceToDevice), "copy");
+        if (S.nnzb)
+            RG_HIP(ctx, hipMemcpy(t->col_ind.p, S.col_ind.p, (size_t)S.nnzb * 4, hipMemcpyDeviceToDevice), "copy");
+        t->nbr = nbr;
+        t->b = b;
+        t->nnzb = S.nnzb;
+        t->is_complex = false;
+    }
+    int it_max = 0;
+    double res_max = 0.0;
+    std::vector<double> hres((size_t)batch);
+    for (int64_t first = 0; first < nbr; first += batch) {
+        const int64_t cnt = std::min<int64_t>(batch, nbr - first);
+        double *Y = (double *)buf[0].p, *Zm = (double *)buf[1].p, *T = (double *)buf[2].p;
+        double *Y2 = (double *)buf[3].p, *Z2 = (double *)buf[4].p;
+        RG_HIP(ctx, launch_patch_gather(first, cnt, b, (const int64_t *)S.row_ptr.p, (const int32_t *)S.col_ind.p,
+                                        (const double *)S.vals64.p, ld, Y, Zm, (double *)cs.p, ctx->stream),
+               "patch gather");
+        bool conv = false;
+        int it = 0;
+        double worst = 0.0;
+        for (; it <= maxit; ++it) {
+            RG_HIP(ctx, hipMemsetAsync(resid.p, 0, (size_t)cnt * 8, ctx->stream), "memset");
+            RG_HIP(ctx, launch_ns_gemm(0, cnt, ld, Zm, Y, T, (double *)resid.p, ctx->stream), "NS gemm");
+            RG_HIP(ctx, hipMemcpyAsync(hres.data(), resid.p, (size_t)cnt * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+            RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+            worst = 0.0;
+            for (int64_t q = 0; q < cnt; ++q) {
+                if (!std::isfinite(hres[q])) { worst = INFINITY; break; }
+                worst = std::max(worst, std::sqrt(hres[q]));
+            }
+            if (worst < tol) { conv = true; break; }
+            // the divergence guard of rsatb/calc/isqrtm.hpp:637-643
+            if (!(worst <= 100.0))
+                return fail(ctx, RG_ERR_INVALID,
+                            "Newton-Schulz diverged on the overlap patches (||I - ZY||_F = " +
+                                std::to_string(worst) + "); is S positive definite?");
+            if (it == maxit) break;
+            RG_HIP(ctx, launch_ns_gemm(1, cnt, ld, Y, T, Y2, nullptr, ctx->stream), "NS gemm");
+            RG_HIP(ctx, launch_ns_gemm(1, cnt, ld, T, Zm, Z2, nullptr, ctx->stream), "NS gemm");
+            std::swap(buf[0], buf[3]);
+            std::swap(buf[1], buf[4]);
+            Y = (double *)buf[0].p;
+            Zm = (double *)buf[1].p;
+            Y2 = (double *)buf[3].p;
+            Z2 = (double *)buf[4].p;
+        }
+        if (!conv)
+            return fail(ctx, RG_ERR_INVALID,
+                        "Newton-Schulz did not reach tol " + std::to_string(tol) + " in " +
+                            std::to_string(maxit) + " iterations (||I - ZY||_F = " + std::to_string(worst) + ")");
+        it_max = std::max(it_max, it);
+        res_max = std::max(res_max, worst);
+        RG_HIP(ctx, launch_patch_extract(first, cnt, b, (const int64_t *)S.row_ptr.p, (const int32_t *)S.col_ind.p, ld,
+                                         (const double *)buf[1].p, (const double *)cs.p, (double *)Z.vals64.p, ctx->stream),
+               "patch extract");
+    }
+    RG_HIP(ctx, launch_bsr_ct_vals(nbr, b, false, S.nnzb, (const int64_t *)Z.row_ptr.p, (const int32_t *)Z.col_ind.p,
+                                   (const double *)Z.vals64.p, (double *)ZH.vals64.p, ctx->stream),
+           "conjugate transpose");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    for (auto &bb : buf) bb.release();
+    Z.present = ZH.present = true;
+    if (iters_out) *iters_out = it_max;
+    if (resid_out) *resid_out = res_max;
+    return RG_OK;
+}
+
+namespace {
+// number of eigenvalues of the symmetric tridiagonal (a, b) below x (Sturm)
+int sturm_count(const std::vector<double> &a, const std::vector<double> &b2, double x) {
+    int cnt = 0;
+    double d = 1.0;
+    for (size_t k = 0; k < a.size(); ++k) {
+        d = a[k] - x - (k ? b2[k] / d : 0.0);
+        if (d == 0.0) d = -1e-300;
+        if (d < 0.0) ++cnt;
+    }
+    return cnt;
+}
+double tridiag_extreme(const std::vector<double> &a, const std::vector<double> &b2, bool want_max) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t k = 0; k < a.size(); ++k) {
+        const double r = (k ? std::sqrt(b2[k]) : 0.0) + (k + 1 < a.size() ? std::sqrt(b2[k + 1]) : 0.0);
+        lo = std::min(lo, a[k] - r);
+        hi = std::max(hi, a[k] + r);
+    }
+    const int target = want_max ? (int)a.size() : 1;
+    for (int it = 0; it < 200 && hi - lo > 1e-14 * std::max(1.0, std::fabs(lo) + std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (sturm_count(a, b2, mid) >= target) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+} // namespace
+
+int rg_estimate_bounds(rg_context *ctx, int form, int steps, uint64_t seed, double padding,
+                       double *emin, double *emax) {
+    if (!ctx || !emin || !emax) return RG_ERR_INVALID;
+    if (form != RG_FORM_ORTHOGONAL && form != RG_FORM_SANDWICH)
+        return fail(ctx, RG_ERR_UNSUPPORTED, "form must be orthogonal or sandwich");
+    if (steps < 2) return fail(ctx, RG_ERR_INVALID, "at least 2 Lanczos steps");
+    const int saved_form = ctx->form;
+    const bool saved_set = ctx->op_set;
+    ctx->form = form;
+    ctx->op_set = true;
+    int rc = check_operator(ctx);
+    ctx->form = saved_form;
+    ctx->op_set = saved_set;
+    if (rc) return rc;
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const int R = 16;
+    VecMode vm;
+    for (int sl = 0; sl < 3; ++sl)
+        vm.complex_vec = vm.complex_vec || (ctx->slots[sl].present && ctx->slots[sl].is_complex);
+    const int64_t rows = op_rows(ctx);
+    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
+    DevBuf v, vp, w, rawl, bsq;
+    RG_HIP(ctx, v.ensure(nb), "hipMalloc");
+    RG_HIP(ctx, vp.ensure(nb), "hipMalloc");
+    RG_HIP(ctx, w.ensure(nb), "hipMalloc");
+    if (form == RG_FORM_SANDWICH) {
+        RG_HIP(ctx, ctx->vec[3].ensure(nb), "hipMalloc");
+        RG_HIP(ctx, ctx->vec[4].ensure(nb), "hipMalloc");
+    }
+    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
+    RG_HIP(ctx, rawl.ensure((size_t)steps * 3 * R * 16), "hipMalloc");
+    RG_HIP(ctx, bsq.ensure((size_t)(steps + 1) * R * 16), "hipMalloc");
+    RG_HIP(ctx, hipMemsetAsync(vp.p, 0, nb, ctx->stream), "memset");
+    RG_HIP(ctx, hipMemsetAsync(bsq.p, 0, (size_t)(steps + 1) * R * 16, ctx->stream), "memset");
+    RG_HIP(ctx, launch_samples(vm.complex_vec ? 1 : 0, seed, rows, R, 1.0 / std::sqrt((double)rows),
+                               vm.complex_vec, v.p, ctx->stream),
+           "start vectors");
+    double2 *rl = (double2 *)rawl.p, *bq = (double2 *)bsq.p;
+    EpiArgs e;
+    e.mode = EPI_FIRST;
+    e.alpha = 1.0;
+    e.beta = 0.0;
+    e.partials = static_cast<double2 *>(ctx->partials.p);
+    const int lg = lanczos_grid(rows, R);
+    void *pv = v.p, *pp = vp.p, *pw = w.p;
+    for (int j = 0; j < steps; ++j) {
+        int g = 0;
+        rc = apply_chain(ctx, vm, pv, pw, R, e, &g, form);
+        if (rc) return rc;
+        rc = reduce(ctx, g, 3, R, rl + (size_t)j * 3 * R);
+        if (rc) return rc;
+        RG_HIP(ctx, launch_lanczos_axpy(rows, R, vm.complex_vec, (double *)pw, (const double *)pv, (const double *)pp,
+                                        rl + (size_t)j * 3 * R + R, bq + (size_t)j * R, (double2 *)ctx->partials.p, lg,
+                                        ctx->stream),
+               "lanczos axpy");
+        rc = reduce(ctx, lg, 1, R, bq + (size_t)(j + 1) * R);
+        if (rc) return rc;
+        RG_HIP(ctx, launch_lanczos_scale(rows, R, vm.complex_vec, (const double *)pw, bq + (size_t)(j + 1) * R, 0.0,
+                                         (double *)pw, ctx->stream),
+               "lanczos scale");
+        void *old = pp;
+        pp = pv;
+        pv = pw;
+        pw = old;
+    }
+    std::vector<double2> hr((size_t)steps * 3 * R), hb((size_t)(steps + 1) * R);
+    RG_HIP(ctx, hipMemcpyAsync(hr.data(), rl, hr.size() * 16, hipMemcpyDeviceToHost, ctx->stream), "download");
+    RG_HIP(ctx, hipMemcpyAsync(hb.data(), bq, hb.size() * 16, hipMemcpyDeviceToHost, ctx->stream), "download");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (ctx->prof) drain_events(ctx);
+    double lo = INFINITY, hi = -INFINITY;
+    for (int c = 0; c < R; ++c) {
+        std::vector<double> a, b2;
+        double scale = 0.0;
+        for (int j = 0; j < steps; ++j) {
+            const double aj = hr[((size_t)j * 3 + 1) * R + c].x;
+            const double bj2 = j ? hb[(size_t)j * R + c].x : 0.0;
+            if (!std::isfinite(aj) || !std::isfinite(bj2)) break;
+            if (j && !(bj2 > 1e-24 * std::max(1.0, scale * scale))) break; // Krylov space exhausted
+            a.push_back(aj);
+            b2.push_back(bj2);
+            scale = std::max(scale, std::fabs(aj) + std::sqrt(bj2));
+        }
+        if (a.empty()) continue;
+        lo = std::min(lo, tridiag_extreme(a, b2, false));
+        hi = std::max(hi, tridiag_extreme(a, b2, true));
+    }
+    if (!std::isfinite(lo) || !std::isfinite(hi) || lo > hi)
+        return fail(ctx, RG_ERR_INVALID, "Automatic bounds estimation did not produce finite energy bounds.");
+    *emin = lo - padding;
+    *emax = hi + padding;
     return RG_OK;
 }
 

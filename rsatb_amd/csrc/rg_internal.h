@@ -47,9 +47,20 @@ struct VecMode {
     bool fp32 = false;        // float storage (precision form)
 };
 
-// Number of workgroups a SpMM launch uses for a matrix with nbr block rows
-// and R columns (the partial-buffer size depends on it).
+// Number of workgroups the block-row-gather (VALU) SpMM uses for nbr block
+// rows and R columns; spmm_max_grid() bounds every SpMM launch's grid (the
+// dot-partial buffer is sized by it).
 int spmm_grid(int64_t nbr, int R);
+int spmm_max_grid();
+// XCD-contiguous row ranges in the MFMA SpMM (default on).
+void set_xcd_map(int on);
+void set_mfma_prefetch(int on);
+void set_mfma_ntcap(int nt);
+
+// Device matrices hold every b x b block column-major; uploads arrive
+// row-major (rsatb's layout) and are transposed on the device.
+hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
+                                  hipStream_t stream);
 
 // Y = epilogue(A X). X, Y: [nbr*b][R] in the storage `vm` describes.
 hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int R,
@@ -71,5 +82,27 @@ hipError_t launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_
 // widening real to complex (zero imaginary part) and/or narrowing to FP32.
 hipError_t launch_stage(const double *in, bool in_complex, void *out, bool out_complex,
                         bool out_fp32, int64_t n, hipStream_t stream);
+
+// ---- graph-local S^{-1/2} patches (rg_isqrt.hip) ----
+hipError_t launch_patch_gather(int64_t first, int64_t count, int b, const int64_t *row_ptr,
+                               const int32_t *col_ind, const double *vals, int ld, double *Y,
+                               double *Z, double *cs, hipStream_t st);
+// mode 0: C = 1.5 I - 0.5 A B and resid[p] += ||I - A B||_F^2; mode 1: C = A B
+hipError_t launch_ns_gemm(int mode, int64_t count, int ld, const double *A, const double *B,
+                          double *C, double *resid, hipStream_t st);
+hipError_t launch_patch_extract(int64_t first, int64_t count, int b, const int64_t *row_ptr,
+                                const int32_t *col_ind, int ld, const double *Z, const double *cs,
+                                double *out, hipStream_t st);
+hipError_t launch_bsr_ct_vals(int64_t nbr, int b, bool cplx, int64_t nnzb, const int64_t *row_ptr,
+                              const int32_t *col_ind, const double *in, double *out,
+                              hipStream_t st);
+
+// ---- Lanczos vector kernels (rg_lanczos.hip) ----
+int lanczos_grid(int64_t rows, int R);
+hipError_t launch_lanczos_axpy(int64_t rows, int R, bool cplx, double *w, const double *v,
+                               const double *vprev, const double2 *alpha, const double2 *bsq,
+                               double2 *partials, int grid, hipStream_t st);
+hipError_t launch_lanczos_scale(int64_t rows, int R, bool cplx, const double *w,
+                                const double2 *bsq, double tiny, double *v, hipStream_t st);
 
 } // namespace rg

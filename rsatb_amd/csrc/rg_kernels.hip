@@ -23,6 +23,7 @@
 // results are deterministic run to run.
 #include "rg_internal.h"
 
+#include <numeric>
 #include <type_traits>
 
 namespace rg {
@@ -30,6 +31,22 @@ namespace {
 
 constexpr int kWaves = 4;   // waves per workgroup
 constexpr int kMaxGrid = 2048;
+
+// XCD-contiguous row ranges in the MFMA SpMM (1) or plain round-robin (0).
+int g_xcd_map = 1;
+// Operand prefetch one K-step ahead in the MFMA SpMM.
+int g_mfma_prefetch = 0;
+// Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
+int g_mfma_ntcap = 16;
+
+// Tiles per wave of the MFMA path for W real columns (0: not eligible).
+__host__ __device__ inline int mfma_nt(int W) {
+    if (W % 32 != 0) return 0;
+    if (W % 256 == 0) return 16;
+    if (W % 128 == 0) return 8;
+    if (W % 64 == 0) return 4;
+    return 2;
+}
 
 template <typename S> struct V2;
 template <> struct V2<double> { using T = double2; };
@@ -156,7 +173,7 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
 #pragma unroll
                 for (int r = 0; r < MAXB; ++r) {
                     if (BS == 0 && r >= bs) continue;
-                    const MT a = blk[r * bs + k];
+                    const MT a = blk[k * bs + r]; // blocks are stored column-major on the device
 #pragma unroll
                     for (int j = 0; j < UPL; ++j) mac(acc[r][j], a, xk[j]);
                 }
@@ -264,6 +281,303 @@ hipError_t by_block(const BsrDev &A, const void *X, void *Y, int R, const EpiArg
     return by_layout<S, CM, CV, 0>(A, X, Y, R, e, grid, st);
 }
 
+
+// ------------------------------------------------------------ MFMA SpMM
+// Dense-block path (b >= 8, real matrix): the block row's stored blocks are
+// concatenated along K, so a row of 17 blocks of 13 x 13 is one 13 x 221
+// panel times a gathered 221 x W panel of X, tiled as 16 x 16 x 4 MFMAs:
+// only the 13 -> 16 row padding is wasted (81 % of issued MFMA work is
+// useful), K is padded once per block row (221 -> 224).
+//
+// Operand layouts of v_mfma_{f64,f32}_16x16x4 (lane l, i = l % 16, g = l / 16):
+//   A[i][k = g], B[k = g][j = i]; accumulator element v holds D[4 g + v][i]
+//   for f32 and D[4 v + g][i] for f64 (both measured on gfx950).
+// N is relabelled so that lane i owns NT *contiguous* real columns
+// col0 + i NT + t (tile t): X, Y and the recurrence operands move as wide
+// vector loads/stores, and the (re, im) halves of a complex column sit in
+// the same lane (tiles 2u, 2u + 1), so the complex dots need no shuffles.
+// Blocks are stored column-major on the device, so the 13 rows of one A
+// column are contiguous and the A fragment load is coalesced.
+template <typename S> struct Acc4;
+template <> struct Acc4<double> { typedef double T __attribute__((ext_vector_type(4))); };
+template <> struct Acc4<float> { typedef float T __attribute__((ext_vector_type(4))); };
+
+__device__ __forceinline__ Acc4<double>::T mfma16(double a, double b, Acc4<double>::T c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ Acc4<float>::T mfma16(float a, float b, Acc4<float>::T c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// NT contiguous elements; NT * sizeof(S) is a multiple of 8 bytes.
+template <typename S, int NT>
+__device__ __forceinline__ void load_row(const S *__restrict__ p, S (&v)[NT]) {
+    if constexpr ((NT * sizeof(S)) % 16 == 0) {
+        const float4 *q = reinterpret_cast<const float4 *>(p);
+        constexpr int NQ = NT * sizeof(S) / 16;
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) {
+            const float4 w = q[t];
+            __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * t, &w, 16);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) v[t] = p[t];
+    }
+}
+template <typename S, int NT>
+__device__ __forceinline__ void store_row(S *__restrict__ p, const S (&v)[NT]) {
+    if constexpr ((NT * sizeof(S)) % 16 == 0) {
+        float4 *q = reinterpret_cast<float4 *>(p);
+        constexpr int NQ = NT * sizeof(S) / 16;
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) {
+            float4 w;
+            __builtin_memcpy(&w, reinterpret_cast<const char *>(v) + 16 * t, 16);
+            q[t] = w;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) p[t] = v[t];
+    }
+}
+
+template <typename S, int NT, int MODE, bool CV, bool PF>
+__global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
+                                                   const int64_t *__restrict__ row_ptr,
+                                                   const int32_t *__restrict__ col_ind,
+                                                   const S *__restrict__ vals,
+                                                   const S *__restrict__ X, S *__restrict__ Y,
+                                                   int W, int R, int xcd_map, EpiArgs e) {
+    using A4 = typename Acc4<S>::T;
+    constexpr int NDOT = MODE == EPI_FIRST ? 3 : MODE == EPI_STEP ? 2 : MODE == EPI_PLAINSTEP ? 1 : 0;
+    constexpr int CW = 16 * NT; // real columns per chunk
+    const S *__restrict__ XS = static_cast<const S *>(e.xs);
+    const S *__restrict__ XP = static_cast<const S *>(e.xp);
+    const S *__restrict__ PSI = static_cast<const S *>(e.psi);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = lane & 15;
+    const int g = lane >> 4;
+    const int nchunks = W / CW; // divides kWaves: a wave keeps chunk wave % nchunks
+    const int col0 = (wave % nchunks) * CW;
+    // Workgroups are dispatched round-robin over the 8 XCDs; with xcd_map each
+    // XCD walks its own contiguous eighth of the block rows, so the rows in
+    // flight on one XCD (and their gathered neighbours) stay close together
+    // in its L2.
+    int64_t r_lo = 0, r_hi = nbr;
+    int64_t kq = blockIdx.x, nk = gridDim.x;
+    if (xcd_map) {
+        const int x = blockIdx.x & 7;
+        kq = blockIdx.x >> 3;
+        nk = gridDim.x >> 3;
+        r_lo = nbr * x / 8;
+        r_hi = nbr * (x + 1) / 8;
+    }
+    const int64_t total = (r_hi - r_lo) * nchunks;
+
+    double dacc[NDOT > 0 ? NDOT : 1][NT];
+#pragma unroll
+    for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
+
+    for (int64_t it = kq * kWaves + wave; it < total; it += nk * kWaves) {
+        const int64_t br = r_lo + it / nchunks;
+        const int64_t s0 = row_ptr[br];
+        const int nblk = (int)(row_ptr[br + 1] - s0);
+        const int nsteps = (nblk * b + 3) >> 2;
+
+        A4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = A4{0, 0, 0, 0};
+
+        // this lane's position k = 4 st + g in the concatenated K range; the
+        // operands of step st + 1 are fetched while step st's MFMAs run
+        int blk = 0, kk = g;
+        while (kk >= b) { kk -= b; ++blk; }
+        auto fetch = [&](int bl, int k, S &a, S (&bv)[NT]) {
+            a = S(0);
+            if (bl < nblk) {
+                const int64_t sb = s0 + bl;
+                if (i < b) a = __builtin_nontemporal_load(vals + sb * (int64_t)(b * b) + k * b + i);
+                const int64_t xr = (int64_t)col_ind[sb] * b + k;
+                load_row<S, NT>(X + xr * W + col0 + i * NT, bv);
+            } else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bv[t] = S(0);
+            }
+        };
+        if constexpr (PF) {
+            S a_cur, bv_cur[NT];
+            fetch(blk, kk, a_cur, bv_cur);
+            for (int st = 0; st < nsteps; ++st) {
+                int kn = kk + 4, bn = blk;
+                while (kn >= b) { kn -= b; ++bn; }
+                S a_nxt, bv_nxt[NT];
+                fetch(bn, kn, a_nxt, bv_nxt);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t] = mfma16(a_cur, bv_cur[t], acc[t]);
+                a_cur = a_nxt;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bv_cur[t] = bv_nxt[t];
+                kk = kn;
+                blk = bn;
+            }
+        } else {
+            for (int st = 0; st < nsteps; ++st) {
+                S a, bv[NT];
+                fetch(blk, kk, a, bv);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
+                kk += 4;
+                while (kk >= b) { kk -= b; ++blk; }
+            }
+        }
+
+        // one output row per pass keeps the epilogue's registers low
+#pragma unroll 1
+        for (int v = 0; v < 4; ++v) {
+            // accumulator element v holds row 4 v + g (f64) or 4 g + v (f32)
+            const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
+            if (r >= b) continue;
+            const int64_t base = (br * b + r) * (int64_t)W + col0 + i * NT;
+            S y[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                // element v of acc[t], selected without dynamic indexing
+                y[t] = v == 0 ? acc[t][0] : v == 1 ? acc[t][1] : v == 2 ? acc[t][2] : acc[t][3];
+            }
+            if constexpr (MODE != EPI_PLAIN) {
+                S xs[NT];
+                load_row<S, NT>(XS + base, xs);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
+                if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP) {
+                    S xp[NT];
+                    load_row<S, NT>(XP + base, xp);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
+                }
+                // dots (FP64): conj(a) b per complex column (t, t+1) or per
+                // real column; a self-dot is real and keeps no imaginary part
+                auto dot = [&](int d, const S (&a)[NT], const S (&bb)[NT], bool self) {
+                    if constexpr (CV) {
+#pragma unroll
+                        for (int t = 0; t < NT; t += 2) {
+                            const double ar = a[t], ai = a[t + 1], br_ = bb[t], bi = bb[t + 1];
+                            dacc[d][t] = __builtin_fma(ar, br_, __builtin_fma(ai, bi, dacc[d][t]));
+                            if (!self)
+                                dacc[d][t + 1] = __builtin_fma(ar, bi, __builtin_fma(-ai, br_, dacc[d][t + 1]));
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            dacc[d][t] = __builtin_fma((double)a[t], (double)bb[t], dacc[d][t]);
+                    }
+                };
+                if constexpr (MODE == EPI_FIRST) {
+                    dot(0, xs, xs, true);
+                    dot(1, xs, y, false);
+                    dot(2, y, y, true);
+                } else if constexpr (MODE == EPI_STEP) {
+                    dot(0, y, y, true);
+                    dot(1, y, xs, false);
+                } else {
+                    S ps[NT];
+                    load_row<S, NT>(PSI + base, ps);
+                    dot(0, ps, y, false);
+                }
+            }
+            store_row<S, NT>(Y + base, y);
+        }
+    }
+
+    if constexpr (NDOT > 0) {
+        // fold the four lane groups (rows of the 16-row tile) of every column
+#pragma unroll
+        for (int d = 0; d < NDOT; ++d)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                dacc[d][t] += __shfl_xor(dacc[d][t], 16);
+                dacc[d][t] += __shfl_xor(dacc[d][t], 32);
+            }
+        // per wave: CW real columns -> CW/2 complex (or CW real) units
+        constexpr int UPC = CV ? CW / 2 : CW;
+        __shared__ double2 sm[kWaves][UPC];
+        for (int d = 0; d < NDOT; ++d) {
+            if (g == 0) {
+#pragma unroll
+                for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
+                    const int uu = CV ? (i * NT + t) / 2 : i * NT + t;
+                    sm[wave][uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
+                }
+            }
+            __syncthreads();
+            for (int u = threadIdx.x; u < R; u += blockDim.x) {
+                const int ch = (u * (CV ? 2 : 1)) / CW;
+                const int uu = u - ch * UPC;
+                double2 acc2 = double2{0.0, 0.0};
+                for (int w = ch; w < kWaves; w += nchunks) {
+                    acc2.x += sm[w][uu].x;
+                    acc2.y += sm[w][uu].y;
+                }
+                e.partials[((int64_t)blockIdx.x * NDOT + d) * R + u] = acc2;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename S, int NT, bool CV, bool PF>
+hipError_t go_mfma_pf(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                      int grid, hipStream_t st) {
+    const int xcd = g_xcd_map;
+    const dim3 gd(grid), bd(64 * kWaves);
+    const S *vals = static_cast<const S *>(A.vals);
+    const S *x = static_cast<const S *>(X);
+    S *y = static_cast<S *>(Y);
+    switch (e.mode) {
+    case EPI_PLAIN:
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
+                           A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+        break;
+    case EPI_FIRST:
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
+                           A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+        break;
+    case EPI_STEP:
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
+                           A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+        break;
+    default:
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, PF>), gd, bd, 0, st, A.nbr,
+                           (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+        break;
+    }
+    return hipGetLastError();
+}
+
+template <typename S, int NT, bool CV>
+hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                   int grid, hipStream_t st) {
+    return g_mfma_prefetch ? go_mfma_pf<S, NT, CV, true>(A, X, Y, W, R, e, grid, st)
+                           : go_mfma_pf<S, NT, CV, false>(A, X, Y, W, R, e, grid, st);
+}
+
+template <typename S, bool CV>
+hipError_t by_nt(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                 int grid, hipStream_t st) {
+    switch (mfma_nt(W)) {
+    case 2: return go_mfma<S, 2, CV>(A, X, Y, W, R, e, grid, st);
+    case 4: return go_mfma<S, 4, CV>(A, X, Y, W, R, e, grid, st);
+    case 8: return go_mfma<S, 8, CV>(A, X, Y, W, R, e, grid, st);
+    default: return go_mfma<S, 16, CV>(A, X, Y, W, R, e, grid, st);
+    }
+}
+
 // ------------------------------------------------------------ reduction
 __global__ __launch_bounds__(256) void k_reduce(const double2 *__restrict__ partials, int ngroups,
                                                 int ndot, int R, double2 *__restrict__ out) {
@@ -351,6 +665,27 @@ __global__ __launch_bounds__(256) void k_stage(const double *__restrict__ in, in
     }
 }
 
+// Row-major -> column-major within every b x b block (real or complex).
+__global__ __launch_bounds__(256) void k_block_transpose(const double *__restrict__ in,
+                                                         double *__restrict__ out, int64_t nnzb,
+                                                         int b, int cplx) {
+    const int64_t per = (int64_t)b * b;
+    const int64_t n = nnzb * per;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = t / per;
+        const int e = (int)(t - s * per);
+        const int r = e / b, c = e - (e / b) * b; // source (r, c) row-major
+        const int64_t dst = s * per + (int64_t)c * b + r;
+        if (cplx) {
+            out[2 * dst] = in[2 * t];
+            out[2 * dst + 1] = in[2 * t + 1];
+        } else {
+            out[dst] = in[t];
+        }
+    }
+}
+
 int grid_1d(int64_t n) {
     int64_t g = (n + 255) / 256;
     if (g > 8192) g = 8192;
@@ -372,9 +707,26 @@ int spmm_grid(int64_t nbr, int R) {
 hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int R,
                        const EpiArgs &e, hipStream_t stream, int *grid_out) {
     if (R < 1 || R > 256 || A.b < 1 || A.b > 16) return hipErrorInvalidValue;
+    if (A.is_complex && !vm.complex_vec) return hipErrorInvalidValue;
+    const int W = R * (vm.complex_vec ? 2 : 1);
+    int nt = mfma_nt(W);
+    while (nt > 2 && nt > g_mfma_ntcap) nt /= 2;
+    const int nchunks = nt > 0 ? W / (16 * nt) : 0;
+    if (!A.is_complex && A.b >= 8 && nt > 0 && kWaves % nchunks == 0) {
+        int64_t g = (A.nbr * nchunks + kWaves - 1) / kWaves;
+        if (g > kMaxGrid) g = kMaxGrid;
+        g = ((g + 7) / 8) * 8; // whole XCD rounds (kMaxGrid is a multiple of 8)
+        const int grid = (int)g;
+        if (grid_out) *grid_out = grid;
+        if (vm.fp32) {
+            return vm.complex_vec ? by_nt<float, true>(A, X, Y, W, R, e, grid, stream)
+                                  : by_nt<float, false>(A, X, Y, W, R, e, grid, stream);
+        }
+        return vm.complex_vec ? by_nt<double, true>(A, X, Y, W, R, e, grid, stream)
+                              : by_nt<double, false>(A, X, Y, W, R, e, grid, stream);
+    }
     const int grid = spmm_grid(A.nbr, R);
     if (grid_out) *grid_out = grid;
-    if (A.is_complex && !vm.complex_vec) return hipErrorInvalidValue;
     if (!vm.fp32) {
         if (A.is_complex) return by_block<double, true, true>(A, X, Y, R, e, grid, stream);
         if (vm.complex_vec) return by_block<double, false, true>(A, X, Y, R, e, grid, stream);
@@ -383,6 +735,19 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
     if (A.is_complex) return by_block<float, true, true>(A, X, Y, R, e, grid, stream);
     if (vm.complex_vec) return by_block<float, false, true>(A, X, Y, R, e, grid, stream);
     return by_block<float, false, false>(A, X, Y, R, e, grid, stream);
+}
+
+int spmm_max_grid() { return kMaxGrid; }
+
+void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
+void set_mfma_prefetch(int on) { g_mfma_prefetch = on ? 1 : 0; }
+void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
+
+hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
+                                  hipStream_t stream) {
+    hipLaunchKernelGGL(k_block_transpose, dim3(grid_1d(nnzb * b * b)), dim3(256), 0, stream, in,
+                       out, nnzb, b, cplx ? 1 : 0);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce(const double2 *partials, int ngroups, int ndot, int R, double2 *out,

@@ -134,37 +134,72 @@ class RSATB:
                 "This model has an overlap: the GPU path iterates the sandwich form Z^H H Z only; "
                 "set PreconditionerOptions(operator_form='sandwich')."
             )
+        if sandwich and o.preconditioner.maxiter != 0:
+            raise RuntimeError(
+                "PreconditionerOptions.maxiter > 0 asks for the sparse Newton-Schulz refinement, which the GPU "
+                "path does not provide; the factor is the graph-local seed (maxiter = 0)."
+            )
         for k in o.kpoints:
             H = assemble(st, self.edges, model, "H", k)
             S = assemble(st, self.edges, model, "S", k) if sandwich else None
             self._kstates.append({"k": tuple(k), "H": H, "S": S, "Z": None})
+        self._bound_k = None
         if o.energy_bounds is not None:
             self._bounds = make_explicit_bounds(*o.energy_bounds)
         elif o.compute_bounds:
-            raise RuntimeError("automatic spectral bounds are not available yet: pass energy_bounds")
+            lo, hi = np.inf, -np.inf
+            for k_idx in range(len(self._kstates)):
+                self._bind_factors(k_idx)
+                e0, e1 = self._ctx.estimate_bounds(self._form(k_idx), steps=self.lanczos_steps,
+                                                   seed=o.sample_seed, padding=0.0)
+                lo, hi = min(lo, e0), max(hi, e1)
+            # padding of EstimateSpectralBounds (rsatb/setup/spectral_bounds.hpp:75, 488)
+            self._bounds = make_explicit_bounds(lo - self.bounds_padding, hi + self.bounds_padding)
         n_rows = st.n_atoms * model.norb
         kind = _lib.SAMPLES_PHASE if o.sample_kind == "phase" else _lib.SAMPLES_RADEMACHER
         self._ctx.set_samples(n_rows, o.nsamples, kind, o.sample_seed)
 
-    def _bind_kpoint(self, idx: int) -> None:
+    lanczos_steps = 200
+    bounds_padding = 0.5
+
+    def _form(self, idx: int) -> int:
+        return _lib.FORM_SANDWICH if self._kstates[idx]["S"] is not None else _lib.FORM_ORTHOGONAL
+
+    def _bind_factors(self, idx: int) -> None:
+        """Puts H(k) (and Z, Z^H for an overlap) on the device."""
+        if self._bound_k == idx:
+            return
         ks = self._kstates[idx]
         self._ctx.upload_bsr(_lib.SLOT_H, ks["H"])
         if ks["S"] is not None:
-            if ks["Z"] is None:
-                raise RuntimeError("the sandwich form needs Z = S^-1/2; set it with set_overlap_factor()")
-            self._ctx.upload_bsr(_lib.SLOT_Z, ks["Z"])
-            self._ctx.upload_bsr(_lib.SLOT_ZH, ks["Z"].conj_transpose())
-            form = _lib.FORM_SANDWICH
-        else:
-            form = _lib.FORM_ORTHOGONAL
+            if ks["Z"] is not None:
+                self._ctx.upload_bsr(_lib.SLOT_Z, ks["Z"])
+                self._ctx.upload_bsr(_lib.SLOT_ZH, ks["Z"].conj_transpose())
+            else:
+                self._ctx.upload_bsr(_lib.SLOT_S, ks["S"])
+                ks["isqrt"] = self._ctx.isqrt_patches()
+        self._bound_k = idx
+
+    def _bind_kpoint(self, idx: int) -> None:
+        self._bind_factors(idx)
         emin, emax = self.energy_bounds()
-        self._ctx.set_operator(form, (emax + emin) / 2.0, (emax - emin) / 2.0)
+        self._ctx.set_operator(self._form(idx), (emax + emin) / 2.0, (emax - emin) / 2.0)
 
     def set_overlap_factor(self, Z: BSR, k_idx: int = 0) -> None:
         """Installs a factor Z (applied as Z x, with Z^H as its adjoint) for a
         k-point, as ISQRTMPreconditioner's borrowing constructor does
-        (rsatb/calc/preconditioner.hpp:194-196)."""
+        (rsatb/calc/preconditioner.hpp:194-196); without one the factor is
+        built on the device from S (rg_isqrt_patches)."""
         self._kstates[k_idx]["Z"] = Z
+        if self._bound_k == k_idx:
+            self._bound_k = None
+
+    def overlap_factor(self, k_idx: int = 0):
+        """The factor Z of a k-point as the device holds it (downloaded)."""
+        self._bind_factors(k_idx)
+        if self._kstates[k_idx]["S"] is None:
+            raise RuntimeError("this model has no overlap")
+        return self._ctx.download_bsr(_lib.SLOT_Z)
 
     # ------------------------------------------------------------- tasks
     def energy_bounds(self) -> tuple[float, float]:
