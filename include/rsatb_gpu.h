@@ -137,6 +137,23 @@ RG_API int rg_bsr_mult(rg_context *ctx, int slot, int R, const double *x, double
 RG_API int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
                    double *mu_cols_out);
 
+/* ---- adjoint (no reference counterpart: the differentiable form) ---- */
+
+/* Keeps the forward states of the next doubled FP64 rg_kpm_moments call for
+ * rg_kpm_moments_backward: psi_0, psi_1 and every pair (psi_{kK-1}, psi_{kK}),
+ * K = stride (stride 1 keeps every state; 0 turns keeping off). */
+RG_API int rg_set_checkpointing(rg_context *ctx, int stride);
+
+/* Reverse mode through the last kept moment run, for a real loss L of the
+ * column-mean moments: dL_dmu[m] = dL/dRe(mu_m), m < M. Writes dL/dH for every
+ * stored entry of the real H in RG_SLOT_H (nnzb*b*b doubles, row-major within
+ * blocks), the window (center, half_width) held fixed; for the sandwich form
+ * Z is held fixed. The adjoint states obey
+ *   lam_n = 2 A~ lam_{n+1} - lam_{n+2} + dL/dpsi_n (direct),
+ * dL/dA~ = lam_1 psi_0^T + 2 sum_{n>=2} lam_n psi_{n-1}^T, summed over columns;
+ * segments between checkpoints are recomputed forward on the device. */
+RG_API int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_out);
+
 /* ---- DOS task (ComputeDos kpm branch, observables.hpp:311-444) ---- */
 
 /* Grid and moment count ComputeDos derives from (emin, emax, dE):

@@ -310,3 +310,37 @@ def isqrt_patches(S) -> Bsr:
         for k in range(ci.size):
             out[s0 + k] = rowblk[:, k * b:(k + 1) * b]
     return Bsr(S.n_block_rows, b, S.row_ptr.copy(), S.col_ind.copy(), out)
+
+
+# ------------------------------------------------- dense moment gradient
+def chebyshev_divided_differences(lam: np.ndarray, m: int) -> np.ndarray:
+    """D[a, b] = (T_m(l_a) - T_m(l_b)) / (l_a - l_b), and T_m'(l_a) on the
+    diagonal, from l = cos(t): a ratio of sines that stays accurate for close
+    eigenvalues."""
+    th = np.arccos(np.clip(lam, -1.0, 1.0))
+    s = 0.5 * (th[:, None] + th[None, :])
+    d = 0.5 * (th[:, None] - th[None, :])
+    num = np.sin(m * s) * np.sin(m * d)
+    den = np.sin(s) * np.sin(d)
+    diag_like = np.abs(d) < 1e-7
+    with np.errstate(divide="ignore", invalid="ignore"):
+        D = num / den
+    # limit d -> 0: m sin(m s) / sin(s) = T_m'(cos s)
+    lim = m * np.sin(m * s) / np.sin(s)
+    return np.where(diag_like, lim, D)
+
+
+def dense_moment_gradient(A: np.ndarray, psi, g: np.ndarray) -> np.ndarray:
+    """dL/dA for L = sum_m g_m Re mu_m, mu_m = mean_c <r_c|T_m(A)|r_c>, A real
+    symmetric with spectrum in (-1, 1): the Daleckii-Krein formula
+    G = V (F o W) V^T, W_ab = mean_c Re(conj(w_a) w_b), w = V^T r,
+    F = sum_m g_m D_m (divided differences of T_m)."""
+    w, V = np.linalg.eigh(0.5 * (A + A.T))
+    pc = _cvec(psi)
+    Wc = V.T @ pc
+    Wt = np.real(np.conj(Wc) @ Wc.T) / pc.shape[1]
+    F = np.zeros_like(Wt)
+    for m, gm in enumerate(g):
+        if gm != 0.0 and m > 0:
+            F += gm * chebyshev_divided_differences(w, m)
+    return V @ (F * Wt) @ V.T

@@ -55,6 +55,8 @@ SIGNATURES = {
     "rg_download_bsr": (_i, [_vp, _i, _pi64, _pi32, _pd]),
     "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
+    "rg_set_checkpointing": (_i, [_vp, _i]),
+    "rg_kpm_moments_backward": (_i, [_vp, _pd, _pd]),
     "rg_cancel": (_i, [_vp]),
     "rg_set_profiling": (_i, [_vp, _i]),
     "rg_kernel_stats": (_i, [_vp, _i, _pd, _pi64, _pd, _pd]),
@@ -269,6 +271,18 @@ class Context:
         )
         _check(self._h, rc, "rg_kpm_moments")
         return (mu, cols) if per_column else mu
+
+    def set_checkpointing(self, stride: int) -> None:
+        _check(self._h, self._lib.rg_set_checkpointing(self._h, int(stride)), "rg_set_checkpointing")
+
+    def kpm_moments_backward(self, dL_dmu: np.ndarray) -> np.ndarray:
+        """dL/dH per stored entry of H ([nnzb, b, b], upload layout)."""
+        info = self.bsr_info(SLOT_H)
+        g = np.ascontiguousarray(np.asarray(dL_dmu, dtype=np.float64))
+        out = np.zeros((info["nnzb"], info["block_dim"], info["block_dim"]))
+        rc = self._lib.rg_kpm_moments_backward(self._h, _dptr(g), _dptr(out))
+        _check(self._h, rc, "rg_kpm_moments_backward")
+        return out
 
     def compute_dos(self, emin: float, emax: float, de: float, precision: int = PREC_FP64):
         npts, _ = dos_plan(emin, emax, de)

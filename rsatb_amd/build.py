@@ -17,7 +17,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librsatb_gpu.so"
 
-SOURCES = ["rg_kernels.hip", "rg_isqrt.hip", "rg_lanczos.hip", "rg_engine.hip"]
+SOURCES = ["rg_kernels.hip", "rg_isqrt.hip", "rg_lanczos.hip", "rg_adjoint.hip", "rg_engine.hip"]
 
 # -packed-fp32-ops: no v_pk_* FP32 arithmetic is generated.
 HIPCC_FLAGS = [

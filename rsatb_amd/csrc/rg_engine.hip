@@ -54,6 +54,9 @@ struct Slot {
     bool is_complex = false;
     DevBuf row_ptr, col_ind, vals64, vals32;
     bool vals32_valid = false;
+    // union lists for the union-staged SpMM (dense blocks, sorted rows)
+    DevBuf grp_ptr, grp_cols, pos;
+    bool has_groups = false;
 };
 
 struct KStat {
@@ -91,6 +94,14 @@ struct rg_context {
     DevBuf partials, raw;
 
     std::atomic<int> cancel{0};
+
+    // forward states kept for rg_kpm_moments_backward
+    int ckpt_stride = 0;
+    DevBuf ckpt;
+    std::vector<int> ckpt_slot; // psi index -> slot in ckpt, -1 when not kept
+    bool ckpt_valid = false;
+    int ckpt_M = 0, ckpt_N = 0, ckpt_R = 0;
+    bool ckpt_complex = false;
 
     bool prof = false;
     KStat stats[RG_KCLASS_COUNT];
@@ -159,6 +170,11 @@ BsrDev dev_view(const Slot &s, bool fp32) {
     d.col_ind = static_cast<const int32_t *>(s.col_ind.p);
     d.vals = fp32 ? s.vals32.p : s.vals64.p;
     d.is_complex = s.is_complex;
+    if (s.has_groups) {
+        d.grp_ptr = static_cast<const int64_t *>(s.grp_ptr.p);
+        d.grp_cols = static_cast<const int32_t *>(s.grp_cols.p);
+        d.pos = static_cast<const int32_t *>(s.pos.p);
+    }
     return d;
 }
 
@@ -170,6 +186,41 @@ double matrix_bytes(const Slot &s, bool fp32) {
 double matrix_flops(const Slot &s, VecMode vm, int R) {
     const double c = s.is_complex ? 4.0 : (vm.complex_vec ? 2.0 : 1.0);
     return 2.0 * (double)s.nnzb * s.b * s.b * (double)R * c;
+}
+
+// Union lists of consecutive groups of block rows (see BsrDev) for a slot
+// with dense blocks and sorted rows; other slots go without.
+int build_groups(rg_context *ctx, Slot &s, const int64_t *rp, const int32_t *ci) {
+    s.has_groups = false;
+    if (s.b < 8 || s.is_complex) return RG_OK;
+    for (int64_t r = 0; r < s.nbr; ++r)
+        for (int64_t q = rp[r] + 1; q < rp[r + 1]; ++q)
+            if (ci[q] <= ci[q - 1]) return RG_OK; // unsorted rows: no union path
+    const int G = union_group_rows();
+    const int64_t ng = (s.nbr + G - 1) / G;
+    std::vector<int64_t> gp((size_t)ng + 1, 0);
+    std::vector<int32_t> gc, pos((size_t)s.nnzb);
+    std::vector<int32_t> u;
+    for (int64_t gi = 0; gi < ng; ++gi) {
+        const int64_t r0 = gi * G, r1 = std::min(s.nbr, r0 + G);
+        u.assign(ci + rp[r0], ci + rp[r1]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (int64_t q = rp[r0]; q < rp[r1]; ++q)
+            pos[q] = (int32_t)(std::lower_bound(u.begin(), u.end(), ci[q]) - u.begin());
+        gc.insert(gc.end(), u.begin(), u.end());
+        gp[gi + 1] = (int64_t)gc.size();
+    }
+    RG_HIP(ctx, s.grp_ptr.ensure(gp.size() * 8), "hipMalloc groups");
+    RG_HIP(ctx, s.grp_cols.ensure(gc.size() * 4 + 4), "hipMalloc groups");
+    RG_HIP(ctx, s.pos.ensure(pos.size() * 4 + 4), "hipMalloc groups");
+    RG_HIP(ctx, hipMemcpy(s.grp_ptr.p, gp.data(), gp.size() * 8, hipMemcpyHostToDevice), "upload groups");
+    if (!gc.empty())
+        RG_HIP(ctx, hipMemcpy(s.grp_cols.p, gc.data(), gc.size() * 4, hipMemcpyHostToDevice), "upload groups");
+    if (!pos.empty())
+        RG_HIP(ctx, hipMemcpy(s.pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice), "upload groups");
+    s.has_groups = true;
+    return RG_OK;
 }
 
 // One SpMM launch with optional profiling; partials go to ctx->partials.
@@ -343,6 +394,7 @@ int rg_create(int device_id, rg_context **out) {
     if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PREFETCH")) set_mfma_prefetch(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
+    if (const char *v = std::getenv("RG_UNION")) set_union_stage(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RG_ERR_HIP;
@@ -365,8 +417,12 @@ void rg_destroy(rg_context *ctx) {
         s.col_ind.release();
         s.vals64.release();
         s.vals32.release();
+        s.grp_ptr.release();
+        s.grp_cols.release();
+        s.pos.release();
     }
     ctx->psi.release();
+    ctx->ckpt.release();
     for (auto &v : ctx->vec) v.release();
     ctx->partials.release();
     ctx->raw.release();
@@ -418,6 +474,8 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     s.b = b;
     s.nnzb = nnzb;
     s.is_complex = (dtype == RG_C128);
+    int rcg = build_groups(ctx, s, row_ptr, col_ind);
+    if (rcg) return rcg;
     s.present = true;
     return RG_OK;
 }
@@ -564,7 +622,33 @@ int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *
     RG_HIP(ctx, ctx->raw.ensure((size_t)nsteps * 3 * R * 16), "hipMalloc moment store");
     double2 *raw = static_cast<double2 *>(ctx->raw.p);
 
-    void *t0 = ctx->vec[0].p, *t1 = ctx->vec[1].p, *t2 = ctx->vec[2].p;
+    // forward states kept for the adjoint: psi_j for j <= 1 and j = 0, K-1 (mod K)
+    const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32;
+    ctx->ckpt_valid = false;
+    int Npsi = 1;
+    for (int n = 1; 2 * n + 1 < M; ++n) ++Npsi;
+    if (keep) {
+        const int K = ctx->ckpt_stride;
+        ctx->ckpt_slot.assign((size_t)Npsi + 1, -1);
+        int nslot = 0;
+        for (int j = 0; j <= Npsi; ++j)
+            if (j <= 1 || K == 1 || j % K == 0 || j % K == K - 1) ctx->ckpt_slot[j] = nslot++;
+        RG_HIP(ctx, ctx->ckpt.ensure((size_t)nslot * nb), "hipMalloc forward checkpoints");
+    }
+    auto kept = [&](int j) -> void * {
+        if (!keep || ctx->ckpt_slot[j] < 0) return nullptr;
+        return static_cast<char *>(ctx->ckpt.p) + (size_t)ctx->ckpt_slot[j] * nb;
+    };
+    // a work buffer that holds neither live state
+    auto scratch = [&](const void *x0, const void *x1) -> void * {
+        for (int i = 0; i < 3; ++i)
+            if (ctx->vec[i].p != x0 && ctx->vec[i].p != x1) return ctx->vec[i].p;
+        return nullptr;
+    };
+
+    void *t0 = kept(0) ? kept(0) : ctx->vec[0].p;
+    void *t1 = kept(1) ? kept(1) : ctx->vec[1].p;
+    void *t2 = ctx->vec[2].p;
     rc = stage_samples(ctx, vm, t0);
     if (rc) return rc;
     if (!doubled) {
@@ -598,16 +682,22 @@ int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *
                 drain_events(ctx);
                 return fail(ctx, RG_ERR_CANCELLED, "interrupted");
             }
+            t2 = kept(n + 1) ? kept(n + 1) : scratch(t0, t1);
             e.xp = t0;
             rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
             if (rc) return rc;
             rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R);
             if (rc) return rc;
             ++nraw;
-            void *sw = t0;
             t0 = t1;
             t1 = t2;
-            t2 = sw;
+        }
+        if (keep) {
+            ctx->ckpt_valid = true;
+            ctx->ckpt_M = M;
+            ctx->ckpt_N = Npsi;
+            ctx->ckpt_R = R;
+            ctx->ckpt_complex = vm.complex_vec;
         }
     } else {
         e.mode = EPI_PLAINSTEP;
@@ -771,6 +861,8 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         t->b = b;
         t->nnzb = S.nnzb;
         t->is_complex = false;
+        int rcg = build_groups(ctx, *t, rp.data(), ci.data());
+        if (rcg) return rcg;
     }
     int it_max = 0;
     double res_max = 0.0;
@@ -952,6 +1044,147 @@ int rg_estimate_bounds(rg_context *ctx, int form, int steps, uint64_t seed, doub
         return fail(ctx, RG_ERR_INVALID, "Automatic bounds estimation did not produce finite energy bounds.");
     *emin = lo - padding;
     *emax = hi + padding;
+    return RG_OK;
+}
+
+int rg_set_checkpointing(rg_context *ctx, int stride) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (stride < 0) return fail(ctx, RG_ERR_INVALID, "checkpoint stride must be >= 0");
+    ctx->ckpt_stride = stride;
+    ctx->ckpt_valid = false;
+    if (stride == 0) ctx->ckpt.release();
+    return RG_OK;
+}
+
+int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_out) {
+    if (!ctx || !dL_dmu || !grad_out) return RG_ERR_INVALID;
+    if (!ctx->ckpt_valid)
+        return fail(ctx, RG_ERR_STATE,
+                    "no forward states: call rg_set_checkpointing(stride > 0), then a doubled FP64 "
+                    "rg_kpm_moments, before the backward pass");
+    int rc = check_operator(ctx);
+    if (rc) return rc;
+    const Slot &Hs = ctx->slots[RG_SLOT_H];
+    if (Hs.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "the backward pass needs a real Hamiltonian");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const int M = ctx->ckpt_M, N = ctx->ckpt_N, R = ctx->ckpt_R, K = ctx->ckpt_stride;
+    const bool sandwich = ctx->form == RG_FORM_SANDWICH;
+    VecMode vm;
+    vm.complex_vec = ctx->ckpt_complex;
+    const int64_t rows = op_rows(ctx);
+    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
+    const int64_t nvec = rows * (int64_t)R * (vm.complex_vec ? 2 : 1);
+    // loss derivatives, zero beyond M
+    std::vector<double> gd((size_t)std::max(M, 2 * N + 3), 0.0);
+    for (int m = 0; m < M; ++m) gd[m] = dL_dmu[m];
+    // mu_1 also enters every odd moment through -mu_1 (dos.hpp:156-159)
+    double G1 = gd[1];
+    for (int n = 2; n <= N; ++n)
+        if (2 * n - 1 < M) G1 -= gd[2 * n - 1];
+
+    // buffers: 3 adjoint states, a zero vector, K + 2 recomputed forward
+    // states, two sandwich transforms, the gradient on H's pattern
+    DevBuf lam[3], zero, seg, zl, zp, grad;
+    for (auto &l : lam) RG_HIP(ctx, l.ensure(nb), "hipMalloc adjoint");
+    RG_HIP(ctx, zero.ensure(nb), "hipMalloc");
+    RG_HIP(ctx, seg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
+    if (sandwich) {
+        RG_HIP(ctx, zl.ensure(nb), "hipMalloc");
+        RG_HIP(ctx, zp.ensure(nb), "hipMalloc");
+        RG_HIP(ctx, ctx->vec[3].ensure(nb), "hipMalloc");
+        RG_HIP(ctx, ctx->vec[4].ensure(nb), "hipMalloc");
+    }
+    const int64_t ng = Hs.nnzb * Hs.b * Hs.b;
+    RG_HIP(ctx, grad.ensure((size_t)ng * 8 + 8), "hipMalloc gradient");
+    RG_HIP(ctx, launch_zero((double *)zero.p, nvec, ctx->stream), "zero");
+    RG_HIP(ctx, launch_zero((double *)grad.p, ng, ctx->stream), "zero");
+    for (auto &l : lam) RG_HIP(ctx, launch_zero((double *)l.p, nvec, ctx->stream), "zero");
+
+    const double a = ctx->half_width, c = ctx->center;
+    auto stored = [&](int j) -> void * {
+        const int sl = ctx->ckpt_slot[j];
+        return sl < 0 ? nullptr : static_cast<char *>(ctx->ckpt.p) + (size_t)sl * nb;
+    };
+    // adjoint states: lam_{n+2}, lam_{n+1}, lam_n rotate through lam[]
+    void *l2 = lam[0].p, *l1 = lam[1].p, *l0 = lam[2].p;
+    const int nseg = N / K;
+    for (int k = nseg; k >= 0; --k) {
+        const int lo = std::max(1, k * K), hi = std::min(k * K + K - 1, N);
+        if (lo > hi) continue;
+        // forward states psi_{lo-1} .. psi_{min(hi+1, N)}: kept or recomputed
+        const int jlo = lo - 1, jhi = std::min(hi + 1, N);
+        std::vector<void *> ptr((size_t)(jhi - jlo + 1), nullptr);
+        int next_seg = 0;
+        EpiArgs ef;
+        ef.mode = EPI_STEP; // its dots land in the partial buffer and are not read
+        ef.alpha = 2.0 / a;
+        ef.beta = -2.0 * c / a;
+        ef.partials = static_cast<double2 *>(ctx->partials.p);
+        for (int j = jlo; j <= jhi; ++j) {
+            void *p = stored(j);
+            if (!p) {
+                if (j < 2 || j - 2 < jlo) return fail(ctx, RG_ERR_STATE, "checkpoint layout broken");
+                p = static_cast<char *>(seg.p) + (size_t)(next_seg++) * nb;
+                ef.xp = ptr[j - 2 - jlo];
+                rc = apply_chain(ctx, vm, ptr[j - 1 - jlo], p, R, ef, nullptr);
+                if (rc) return rc;
+            }
+            ptr[j - jlo] = p;
+        }
+        for (int n = hi; n >= lo; --n) {
+            if (ctx->cancel.load()) {
+                (void)hipStreamSynchronize(ctx->stream);
+                drain_events(ctx);
+                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+            }
+            EpiArgs e;
+            e.mode = EPI_ADJ;
+            e.alpha = 2.0 / a;
+            e.beta = -2.0 * c / a;
+            e.xp = l2;
+            e.qz = ptr[n - jlo];
+            e.cz = (2 * n < M) ? 4.0 * gd[2 * n] / R : 0.0;
+            e.qm = ptr[n - 1 - jlo];
+            e.cm = (n >= 2) ? ((2 * n - 1 < M) ? 2.0 * gd[2 * n - 1] / R : 0.0) : G1 / R;
+            const bool has_next = n + 1 <= N;
+            e.qp = has_next ? ptr[n + 1 - jlo] : zero.p;
+            e.cp = (has_next && 2 * n + 1 < M) ? 2.0 * gd[2 * n + 1] / R : 0.0;
+            // lam_n = 2 A~ lam_{n+1} - lam_{n+2} + d_n  (lam_{N+1} = lam_{N+2} = 0)
+            rc = apply_chain(ctx, vm, l1, l0, R, e, nullptr);
+            if (rc) return rc;
+            // dL/dA~ += coef lam_n psi_{n-1}^T (1 for psi_1 = A~ psi_0, 2 after)
+            const double coef = (n >= 2 ? 2.0 : 1.0) / a;
+            const void *u = l0, *v = ptr[n - 1 - jlo];
+            if (sandwich) { // dL/dH = Z (dL/dA) Z^H
+                EpiArgs pl;
+                pl.mode = EPI_PLAIN;
+                rc = spmm(ctx, RG_SLOT_Z, vm, l0, zl.p, R, pl, nullptr);
+                if (rc) return rc;
+                rc = spmm(ctx, RG_SLOT_Z, vm, v, zp.p, R, pl, nullptr);
+                if (rc) return rc;
+                u = zl.p;
+                v = zp.p;
+            }
+            RG_HIP(ctx, launch_grad_accum(Hs.nbr, Hs.b, Hs.nnzb, (const int64_t *)Hs.row_ptr.p,
+                                          (const int32_t *)Hs.col_ind.p, (const double *)u,
+                                          (const double *)v, R, vm.complex_vec, coef,
+                                          (double *)grad.p, ctx->stream),
+                   "gradient accumulation");
+            void *old = l2;
+            l2 = l1;
+            l1 = l0;
+            l0 = old;
+        }
+    }
+    std::vector<double> cm((size_t)ng);
+    RG_HIP(ctx, hipMemcpyAsync(cm.data(), grad.p, (size_t)ng * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (ctx->prof) drain_events(ctx);
+    const int b = Hs.b;
+    for (int64_t q = 0; q < Hs.nnzb; ++q)
+        for (int r = 0; r < b; ++r)
+            for (int cc = 0; cc < b; ++cc)
+                grad_out[(q * b + r) * b + cc] = cm[(q * b + cc) * b + r];
     return RG_OK;
 }
 

@@ -16,6 +16,12 @@ struct BsrDev {
     const int32_t *col_ind = nullptr;
     const void *vals = nullptr; // double / double2 / float / float2
     bool is_complex = false;
+    // union lists of consecutive groups of union_group_rows() block rows
+    // (group g's sorted distinct block columns are grp_cols[grp_ptr[g] ..
+    // grp_ptr[g+1]); pos[s] is block s's index in its group's list), or null
+    const int64_t *grp_ptr = nullptr;
+    const int32_t *grp_cols = nullptr;
+    const int32_t *pos = nullptr;
 };
 
 // Epilogue of a SpMM launch. y = sum_blocks A x is finished as
@@ -23,10 +29,13 @@ struct BsrDev {
 //   EPI_FIRST      Y = alpha y + beta Xs;           dots <Xs|Xs>, <Xs|Y>, <Y|Y>
 //   EPI_STEP       Y = alpha y + beta Xs - Xp;      dots <Y|Y>, <Y|Xs>
 //   EPI_PLAINSTEP  Y = alpha y + beta Xs - Xp;      dot  <Psi|Y>
+//   EPI_ADJ        Y = alpha y + beta Xs - Xp + cm Qm + cz Qz + cp Qp (no dots):
+//                  one step of the adjoint recurrence, the Q vectors being the
+//                  forward states psi_{n-1}, psi_n, psi_{n+1}
 // Dots are per sample column, conjugate-linear in the left argument, and
 // leave one (re, im) FP64 partial per workgroup and column in `partials`
 // laid out [gridDim.x][ndot][R].
-enum Epi { EPI_PLAIN = 0, EPI_FIRST = 1, EPI_STEP = 2, EPI_PLAINSTEP = 3 };
+enum Epi { EPI_PLAIN = 0, EPI_FIRST = 1, EPI_STEP = 2, EPI_PLAINSTEP = 3, EPI_ADJ = 4 };
 
 inline int epi_ndot(int mode) {
     return mode == EPI_FIRST ? 3 : mode == EPI_STEP ? 2 : mode == EPI_PLAINSTEP ? 1 : 0;
@@ -39,6 +48,8 @@ struct EpiArgs {
     const void *xp = nullptr;
     const void *psi = nullptr;
     double2 *partials = nullptr;
+    const void *qm = nullptr, *qz = nullptr, *qp = nullptr; // EPI_ADJ
+    double cm = 0.0, cz = 0.0, cp = 0.0;
 };
 
 // Vector storage of the moment loop.
@@ -56,6 +67,8 @@ int spmm_max_grid();
 void set_xcd_map(int on);
 void set_mfma_prefetch(int on);
 void set_mfma_ntcap(int nt);
+void set_union_stage(int on);
+int union_group_rows();
 
 // Device matrices hold every b x b block column-major; uploads arrive
 // row-major (rsatb's layout) and are transposed on the device.
@@ -104,5 +117,14 @@ hipError_t launch_lanczos_axpy(int64_t rows, int R, bool cplx, double *w, const 
                                double2 *partials, int grid, hipStream_t st);
 hipError_t launch_lanczos_scale(int64_t rows, int R, bool cplx, const double *w,
                                 const double2 *bsq, double tiny, double *v, hipStream_t st);
+
+// ---- adjoint (rg_adjoint.hip) ----
+// G[e] += coef * sum_c Re(U[i, c] conj(V[j, c])) for every stored entry e =
+// (i, j) of the pattern (row_ptr, col_ind, b); G in column-major block layout.
+hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
+                             const int32_t *col_ind, const double *U, const double *V, int R,
+                             bool cplx, double coef, double *G, hipStream_t st);
+// y = 0 over n doubles
+hipError_t launch_zero(double *y, int64_t n, hipStream_t st);
 
 } // namespace rg

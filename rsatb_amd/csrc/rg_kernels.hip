@@ -23,6 +23,7 @@
 // results are deterministic run to run.
 #include "rg_internal.h"
 
+#include <algorithm>
 #include <numeric>
 #include <type_traits>
 
@@ -38,6 +39,10 @@ int g_xcd_map = 1;
 int g_mfma_prefetch = 0;
 // Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
 int g_mfma_ntcap = 16;
+// Union-staged path for dense blocks (when the matrix carries group lists).
+int g_union_stage = 1;
+// LDS bytes one union-staged workgroup stages per chunk (two groups per CU).
+constexpr size_t kGrpLdsBudget = 72 * 1024;
 
 // Tiles per wave of the MFMA path for W real columns (0: not eligible).
 __host__ __device__ inline int mfma_nt(int W) {
@@ -204,6 +209,11 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
                 } else if (mode == EPI_PLAINSTEP) {
                     y = sub(affine(y, e.alpha, e.beta, XS[idx]), XP[idx]);
                     dotacc(dacc[0][j], PSI[idx], y);
+                } else if (mode == EPI_ADJ) {
+                    y = sub(affine(y, e.alpha, e.beta, XS[idx]), XP[idx]);
+                    y = affine(y, 1.0, e.cm, static_cast<const VT *>(e.qm)[idx]);
+                    y = affine(y, 1.0, e.cz, static_cast<const VT *>(e.qz)[idx]);
+                    y = affine(y, 1.0, e.cp, static_cast<const VT *>(e.qp)[idx]);
                 }
                 Y[idx] = y;
             }
@@ -455,11 +465,24 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
                     y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
-                if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP) {
+                if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
                     S xp[NT];
                     load_row<S, NT>(XP + base, xp);
 #pragma unroll
                     for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
+                }
+                if constexpr (MODE == EPI_ADJ) {
+                    const S *qs[3] = {static_cast<const S *>(e.qm), static_cast<const S *>(e.qz),
+                                      static_cast<const S *>(e.qp)};
+                    const double cs[3] = {e.cm, e.cz, e.cp};
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        S qv[NT];
+                        load_row<S, NT>(qs[q] + base, qv);
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
+                    }
                 }
                 // dots (FP64): conj(a) b per complex column (t, t+1) or per
                 // real column; a self-dot is real and keeps no imaginary part
@@ -485,7 +508,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                 } else if constexpr (MODE == EPI_STEP) {
                     dot(0, y, y, true);
                     dot(1, y, xs, false);
-                } else {
+                } else if constexpr (MODE == EPI_PLAINSTEP) {
                     S ps[NT];
                     load_row<S, NT>(PSI + base, ps);
                     dot(0, ps, y, false);
@@ -552,8 +575,12 @@ hipError_t go_mfma_pf(const BsrDev &A, const void *X, void *Y, int W, int R, con
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
-    default:
+    case EPI_PLAINSTEP:
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, PF>), gd, bd, 0, st, A.nbr,
+                           (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+        break;
+    default:
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV, PF>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     }
@@ -578,7 +605,271 @@ hipError_t by_nt(const BsrDev &A, const void *X, void *Y, int W, int R, const Ep
     }
 }
 
+// ---------------------------------------------- union-staged MFMA SpMM
+// The 8 block rows of a workgroup (one wave each) gather, between them, far
+// fewer distinct X block panels than 8 x 17: consecutive diamond atoms share
+// neighbours, so a group of 8 needs ~56 distinct panels instead of 136. The
+// group's union list (sorted block columns, host-built once per matrix) is
+// streamed through LDS C panels at a time; each wave runs its own blocks that
+// fall in the staged chunk (K concatenated within the chunk) with its B
+// fragments read from LDS. Two workgroups per CU overlap one group's staging
+// with the other's MFMAs. Every wave owns all W columns (NT = W / 16).
+// LDS rows are swizzled in 16-byte units so that the 16 lanes reading one K
+// row (64 contiguous bytes each) hit distinct banks.
+constexpr int kGrpWaves = 8;
+
+template <typename S, int NT>
+__device__ __forceinline__ int grp_swz(int u) { // logical -> physical 16-byte unit
+    constexpr int NQ = NT * (int)sizeof(S) / 16;
+    if constexpr (NQ <= 1) {
+        return u;
+    } else {
+        const int i = u / NQ, q = u - (u / NQ) * NQ;
+        return i * NQ + (q ^ ((i * NQ / 16) & (NQ - 1)));
+    }
+}
+
+template <typename S, int NT, int MODE, bool CV>
+__global__ __launch_bounds__(512) void k_spmm_grp(int64_t nbr, int b,
+                                                  const int64_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col_ind,
+                                                  const S *__restrict__ vals,
+                                                  const int64_t *__restrict__ grp_ptr,
+                                                  const int32_t *__restrict__ grp_cols,
+                                                  const int32_t *__restrict__ pos,
+                                                  const S *__restrict__ X, S *__restrict__ Y,
+                                                  int R, int C, int xcd_map, EpiArgs e) {
+    using A4 = typename Acc4<S>::T;
+    constexpr int NDOT = MODE == EPI_FIRST ? 3 : MODE == EPI_STEP ? 2 : MODE == EPI_PLAINSTEP ? 1 : 0;
+    constexpr int W = 16 * NT;                 // real columns, all in one wave
+    constexpr int UW = W * (int)sizeof(S) / 16; // 16-byte units per staged row
+    extern __shared__ float4 lds_units[];
+    const S *__restrict__ XS = static_cast<const S *>(e.xs);
+    const S *__restrict__ XP = static_cast<const S *>(e.xp);
+    const S *__restrict__ PSI = static_cast<const S *>(e.psi);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = lane & 15;
+    const int g = lane >> 4;
+    const int64_t ngroups = (nbr + kGrpWaves - 1) / kGrpWaves;
+    // XCD x walks the x-th contiguous eighth of the groups
+    int64_t g_lo = 0, g_hi = ngroups, kq = blockIdx.x, nk = gridDim.x;
+    if (xcd_map) {
+        const int64_t per = (ngroups + 7) / 8;
+        const int x = blockIdx.x & 7;
+        kq = blockIdx.x >> 3;
+        nk = gridDim.x >> 3;
+        g_lo = x * per;
+        g_hi = min(ngroups, (int64_t)(x + 1) * per);
+    }
+
+    double dacc[NDOT > 0 ? NDOT : 1][NT];
+#pragma unroll
+    for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
+
+    for (int64_t grp = g_lo + kq; grp < g_hi; grp += nk) {
+        const int64_t br = grp * kGrpWaves + wave;
+        const bool has_row = br < nbr;
+        const int64_t u0 = grp_ptr[grp];
+        const int nu = (int)(grp_ptr[grp + 1] - u0);
+        int64_t sp = has_row ? row_ptr[br] : 0;
+        const int64_t s1 = has_row ? row_ptr[br + 1] : 0;
+
+        A4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = A4{0, 0, 0, 0};
+
+        for (int cu = 0; cu < nu; cu += C) {
+            const int nc = min(C, nu - cu);
+            // stage panels cu .. cu + nc - 1: b rows of W elements each
+            const int units = nc * b * UW;
+            for (int t = threadIdx.x; t < units; t += blockDim.x) {
+                const int rowl = t / UW, uu = t - rowl * UW;
+                const int j = rowl / b, kk = rowl - j * b;
+                const int64_t xr = (int64_t)grp_cols[u0 + cu + j] * b + kk;
+                const float4 v = reinterpret_cast<const float4 *>(X + xr * W)[uu];
+                lds_units[rowl * UW + grp_swz<S, NT>(uu)] = v;
+            }
+            __syncthreads();
+            if (has_row) {
+                int64_t se = sp;
+                while (se < s1 && pos[se] < cu + nc) ++se;
+                const int nk_ = (int)(se - sp) * b;
+                const int nsteps = (nk_ + 3) >> 2;
+                int blk = 0, kk = g;
+                while (kk >= b) { kk -= b; ++blk; }
+                for (int st = 0; st < nsteps; ++st) {
+                    S a = S(0);
+                    S bv[NT];
+                    if (4 * st + g < nk_) {
+                        const int64_t sb = sp + blk;
+                        if (i < b) a = __builtin_nontemporal_load(vals + sb * (int64_t)(b * b) + kk * b + i);
+                        const int rowl = (pos[sb] - cu) * b + kk;
+                        constexpr int NQ = NT * (int)sizeof(S) / 16;
+                        const float4 *src = lds_units + rowl * UW;
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const float4 w = src[grp_swz<S, NT>(i * NQ + q)];
+                            __builtin_memcpy(reinterpret_cast<char *>(bv) + 16 * q, &w, 16);
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) bv[t] = S(0);
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
+                    kk += 4;
+                    while (kk >= b) { kk -= b; ++blk; }
+                }
+                sp = se;
+            }
+            __syncthreads();
+        }
+
+        if (!has_row) continue;
+#pragma unroll 1
+        for (int v = 0; v < 4; ++v) {
+            const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
+            if (r >= b) continue;
+            const int64_t base = (br * b + r) * (int64_t)W + i * NT;
+            S y[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                y[t] = v == 0 ? acc[t][0] : v == 1 ? acc[t][1] : v == 2 ? acc[t][2] : acc[t][3];
+            if constexpr (MODE != EPI_PLAIN) {
+                S xs[NT];
+                load_row<S, NT>(XS + base, xs);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
+                if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
+                    S xp[NT];
+                    load_row<S, NT>(XP + base, xp);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
+                }
+                if constexpr (MODE == EPI_ADJ) {
+                    const S *qs[3] = {static_cast<const S *>(e.qm), static_cast<const S *>(e.qz),
+                                      static_cast<const S *>(e.qp)};
+                    const double cs[3] = {e.cm, e.cz, e.cp};
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        S qv[NT];
+                        load_row<S, NT>(qs[q] + base, qv);
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
+                    }
+                }
+                auto dot = [&](int d, const S (&a)[NT], const S (&bb)[NT], bool self) {
+                    if constexpr (CV) {
+#pragma unroll
+                        for (int t = 0; t < NT; t += 2) {
+                            const double ar = a[t], ai = a[t + 1], br_ = bb[t], bi = bb[t + 1];
+                            dacc[d][t] = __builtin_fma(ar, br_, __builtin_fma(ai, bi, dacc[d][t]));
+                            if (!self)
+                                dacc[d][t + 1] = __builtin_fma(ar, bi, __builtin_fma(-ai, br_, dacc[d][t + 1]));
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            dacc[d][t] = __builtin_fma((double)a[t], (double)bb[t], dacc[d][t]);
+                    }
+                };
+                if constexpr (MODE == EPI_FIRST) {
+                    dot(0, xs, xs, true);
+                    dot(1, xs, y, false);
+                    dot(2, y, y, true);
+                } else if constexpr (MODE == EPI_STEP) {
+                    dot(0, y, y, true);
+                    dot(1, y, xs, false);
+                } else if constexpr (MODE == EPI_PLAINSTEP) {
+                    S ps[NT];
+                    load_row<S, NT>(PSI + base, ps);
+                    dot(0, ps, y, false);
+                }
+            }
+            store_row<S, NT>(Y + base, y);
+        }
+    }
+
+    if constexpr (NDOT > 0) {
+#pragma unroll
+        for (int d = 0; d < NDOT; ++d)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                dacc[d][t] += __shfl_xor(dacc[d][t], 16);
+                dacc[d][t] += __shfl_xor(dacc[d][t], 32);
+            }
+        constexpr int UPC = CV ? W / 2 : W;
+        double2 *sm = reinterpret_cast<double2 *>(lds_units); // staging area is free now
+        for (int d = 0; d < NDOT; ++d) {
+            if (g == 0) {
+#pragma unroll
+                for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
+                    const int uu = CV ? (i * NT + t) / 2 : i * NT + t;
+                    sm[wave * UPC + uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
+                }
+            }
+            __syncthreads();
+            for (int u = threadIdx.x; u < R; u += blockDim.x) {
+                double2 a2 = double2{0.0, 0.0};
+                for (int w = 0; w < kGrpWaves; ++w) {
+                    a2.x += sm[w * UPC + u].x;
+                    a2.y += sm[w * UPC + u].y;
+                }
+                e.partials[((int64_t)blockIdx.x * NDOT + d) * R + u] = a2;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename S, int NT, bool CV>
+hipError_t go_grp(const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, int grid,
+                  int C, size_t lds, hipStream_t st) {
+    const dim3 gd(grid), bd(64 * kGrpWaves);
+    const S *vals = static_cast<const S *>(A.vals);
+    const S *x = static_cast<const S *>(X);
+    S *y = static_cast<S *>(Y);
+    const int xcd = g_xcd_map;
+#define RG_GRP_LAUNCH(MODE)                                                                        \
+    hipLaunchKernelGGL((k_spmm_grp<S, NT, MODE, CV>), gd, bd, lds, st, A.nbr, (int)A.b, A.row_ptr,  \
+                       A.col_ind, vals, A.grp_ptr, A.grp_cols, A.pos, x, y, R, C, xcd, e)
+    switch (e.mode) {
+    case EPI_PLAIN: RG_GRP_LAUNCH(EPI_PLAIN); break;
+    case EPI_FIRST: RG_GRP_LAUNCH(EPI_FIRST); break;
+    case EPI_STEP: RG_GRP_LAUNCH(EPI_STEP); break;
+    case EPI_PLAINSTEP: RG_GRP_LAUNCH(EPI_PLAINSTEP); break;
+    default: RG_GRP_LAUNCH(EPI_ADJ); break;
+    }
+#undef RG_GRP_LAUNCH
+    return hipGetLastError();
+}
+
+template <typename S, bool CV>
+hipError_t grp_by_nt(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                     int grid, int C, size_t lds, hipStream_t st) {
+    switch (W / 16) {
+    case 2: return go_grp<S, 2, CV>(A, X, Y, R, e, grid, C, lds, st);
+    case 4: return go_grp<S, 4, CV>(A, X, Y, R, e, grid, C, lds, st);
+    case 8: return go_grp<S, 8, CV>(A, X, Y, R, e, grid, C, lds, st);
+    default: return go_grp<S, 16, CV>(A, X, Y, R, e, grid, C, lds, st);
+    }
+}
+
 // ------------------------------------------------------------ reduction
+// Neumaier-compensated sum of a and b into (s, c): s + c carries the exact
+// sum to twice the working precision (Neumaier 1974, "improved Kahan").
+__device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+}
+
 __global__ __launch_bounds__(256) void k_reduce(const double2 *__restrict__ partials, int ngroups,
                                                 int ndot, int R, double2 *__restrict__ out) {
     const int lane = threadIdx.x & 63;
@@ -587,18 +878,23 @@ __global__ __launch_bounds__(256) void k_reduce(const double2 *__restrict__ part
     if (pair >= npairs) return;
     const int d = pair / R;
     const int u = pair - d * R;
-    double sx = 0.0, sy = 0.0;
+    // compensated FP64 sums over the workgroup partials, then a fixed xor tree
+    double sx = 0.0, cx = 0.0, sy = 0.0, cy = 0.0;
     for (int gi = lane; gi < ngroups; gi += 64) {
         const double2 p = partials[((int64_t)gi * ndot + d) * R + u];
-        sx += p.x;
-        sy += p.y;
+        two_sum_acc(sx, cx, p.x);
+        two_sum_acc(sy, cy, p.y);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-        sx += __shfl_xor(sx, off);
-        sy += __shfl_xor(sy, off);
+        const double ox = __shfl_xor(sx, off), ocx = __shfl_xor(cx, off);
+        const double oy = __shfl_xor(sy, off), ocy = __shfl_xor(cy, off);
+        cx += ocx;
+        cy += ocy;
+        two_sum_acc(sx, cx, ox);
+        two_sum_acc(sy, cy, oy);
     }
-    if (lane == 0) out[pair] = double2{sx, sy};
+    if (lane == 0) out[pair] = double2{sx + cx, sy + cy};
 }
 
 // --------------------------------------------------------------- samples
@@ -709,6 +1005,29 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
     if (R < 1 || R > 256 || A.b < 1 || A.b > 16) return hipErrorInvalidValue;
     if (A.is_complex && !vm.complex_vec) return hipErrorInvalidValue;
     const int W = R * (vm.complex_vec ? 2 : 1);
+    const size_t elem = vm.fp32 ? 4 : 8;
+    // each lane owns W/16 columns; they must fill whole 16-byte LDS units
+    if (g_union_stage && !A.is_complex && A.b >= 8 && A.grp_ptr &&
+        (W == 32 || W == 64 || W == 128 || W == 256) && W * elem >= 256) {
+        const size_t panel = (size_t)A.b * W * elem;
+        int C = (int)(kGrpLdsBudget / panel);
+        if (C > 16) C = 16;
+        if (C >= 1) {
+            const int64_t ngroups = (A.nbr + kGrpWaves - 1) / kGrpWaves;
+            int64_t g = g_xcd_map ? ((ngroups + 7) / 8) * 8 : ngroups;
+            if (g > kMaxGrid) g = kMaxGrid; // grid-stride over groups (a multiple of 8)
+            {
+                const int grid = (int)g;
+                if (grid_out) *grid_out = grid;
+                const size_t lds = std::max((size_t)C * panel, (size_t)kGrpWaves * (W / (vm.complex_vec ? 2 : 1)) * 16);
+                if (vm.fp32)
+                    return vm.complex_vec ? grp_by_nt<float, true>(A, X, Y, W, R, e, grid, C, lds, stream)
+                                          : grp_by_nt<float, false>(A, X, Y, W, R, e, grid, C, lds, stream);
+                return vm.complex_vec ? grp_by_nt<double, true>(A, X, Y, W, R, e, grid, C, lds, stream)
+                                      : grp_by_nt<double, false>(A, X, Y, W, R, e, grid, C, lds, stream);
+            }
+        }
+    }
     int nt = mfma_nt(W);
     while (nt > 2 && nt > g_mfma_ntcap) nt /= 2;
     const int nchunks = nt > 0 ? W / (16 * nt) : 0;
@@ -742,6 +1061,8 @@ int spmm_max_grid() { return kMaxGrid; }
 void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
 void set_mfma_prefetch(int on) { g_mfma_prefetch = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
+void set_union_stage(int on) { g_union_stage = on ? 1 : 0; }
+int union_group_rows() { return kGrpWaves; }
 
 hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
                                   hipStream_t stream) {
