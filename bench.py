@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--moments", type=int, default=2048)
     p.add_argument("--cpu-moments", type=int, default=32, help="moments in the timed CPU sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward leg")
+    p.add_argument("--adj-cells", type=int, default=128, help="graphene rectangular cells per edge (adjoint leg)")
     return p.parse_args()
 
 
@@ -98,6 +101,63 @@ def cpu_baseline(calc, n_moments: int, ref_mu) -> dict:
             "sample": f"same operator and samples, first {n_moments} moments ({n_moments // 2} sandwich applies, "
                       f"doubled), fused FP64 oracle loop, {dt:.2f} s",
             "seconds": round(dt, 3), "max_abs_moment_diff_vs_gpu": err}
+
+
+def dos_error(mu_ref, mu):
+    """max_E |rho - rho_ref| / max_E rho_ref on 2001 points of [-1, 1]."""
+    import numpy as np
+
+    from rsatb_amd import _lib
+
+    x = -1.0 + np.arange(2001) / 1000.0
+    r_ref = _lib.kpm_density(mu_ref, x, 0.5 / 1000.0)
+    r = _lib.kpm_density(mu, x, 0.5 / 1000.0)
+    return float(np.max(np.abs(r - r_ref)) / np.max(r_ref))
+
+
+def adjoint_leg(cells: int, steps: int) -> dict:
+    """Backward/forward time of the differentiable moments on graphene
+    (cells x cells rectangular, onsite + 3 hopping shells, NN overlap 0.1 with
+    Z held fixed), R = 32, M = 512, checkpoints every 32 steps."""
+    import numpy as np
+
+    from rsatb_amd import _lib
+    from rsatb_amd.autograd import DifferentiableKPM, sk_pz_parameterization
+    from rsatb_amd.lattice import graphene_rect_supercell, neighbor_edges
+    from rsatb_amd.models import SkPzModel, assemble
+
+    st = graphene_rect_supercell(cells, cells)
+    e = neighbor_edges(st, 2.9)
+    model = sk_pz_parameterization(st, e, [1.6, 2.5, 2.9])
+    theta = np.array([0.0, -2.7, -0.2, -0.18])
+    S = assemble(st, neighbor_edges(st, 1.6), SkPzModel(overlap_shells=[(1.6, 0.1, 0.0)]), "S")
+    with _lib.Context(0) as c:
+        c.upload_bsr(_lib.SLOT_S, S)
+        c.isqrt_patches()
+        Z = c.download_bsr(_lib.SLOT_Z)
+        c.upload_bsr(_lib.SLOT_H, model.build(theta))
+        lo, hi = c.estimate_bounds(_lib.FORM_SANDWICH, steps=200, padding=0.5)
+    M = 512
+    eng = DifferentiableKPM(model, 32, (lo, hi), M, checkpoint_stride=32, sample_kind="phase", Z=Z)
+    g = np.linspace(1.0, -1.0, M)
+    eng.forward(theta)
+    eng.backward(g)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.forward(theta)
+    t_f = (time.perf_counter() - t0) / steps
+    t_b = 0.0
+    for _ in range(steps):
+        eng.forward(theta)
+        t1 = time.perf_counter()
+        grad = eng.backward(g)
+        t_b += time.perf_counter() - t1
+    t_b /= steps
+    eng.ctx.close()
+    return {"workload": f"graphene {cells}x{cells} rectangular ({st.n_atoms} atoms), onsite + 3 hopping shells, "
+                        "NN overlap 0.1 (Z fixed), R=32 phase, M=512, checkpoints every 32 steps, FP64",
+            "forward_ms": round(t_f * 1e3, 3), "backward_ms": round(t_b * 1e3, 3),
+            "bwd_fwd": round(t_b / t_f, 3), "grad_theta": [float(x) for x in grad]}
 
 
 def main() -> int:
@@ -156,6 +216,18 @@ def main() -> int:
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "count": v["count"]} for k, v in stats.items()},
         "mu0": float(mu[0].real), "max_abs_mu": float(np.max(np.abs(mu))),
     }
+    if not args.no_fp32:
+        ctx.kpm_moments(M, _lib.PREC_FP32)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mu32 = ctx.kpm_moments(M, _lib.PREC_FP32)
+        dt32 = time.perf_counter() - t0
+        v32 = rows * args.nsamples * M / (dt32 / args.steps)
+        line["fp32"] = {"value": v32, "ms_per_step": dt32 / args.steps * 1e3, "speedup_vs_fp64": v32 / value,
+                        "max_abs_dmu": float(np.max(np.abs(mu32 - mu))), "dos_err": dos_error(mu, mu32),
+                        "dos_err_definition": "max_E |rho32 - rho64| / max_E rho64, Jackson, 2001 points"}
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(args.adj_cells, max(1, args.steps))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(calc, args.cpu_moments, mu)
     print(json.dumps(line), flush=True)

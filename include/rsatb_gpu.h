@@ -137,6 +137,22 @@ RG_API int rg_bsr_mult(rg_context *ctx, int slot, int R, const double *x, double
 RG_API int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
                    double *mu_cols_out);
 
+/* ---- projected moments (kpm_pdos_moments multivector, pdos.hpp:149-266) ---- */
+
+/* The moments of rg_kpm_moments resolved onto groups of rows from the same
+ * recurrence: mu_m^g = mean over columns of <t|P_g|t'>, P_g the diagonal
+ * projector on the rows r with group_of_row[r] == g (a negative id puts a row
+ * in no group; ids must be < n_groups, pdos.hpp:159-167). n_rows must equal
+ * the sample rows. Doubling as rg_kpm_moments, applied per group (the masked
+ * identity of pdos.hpp:29-36).
+ * total_out: [M] complex, the all-rows moments (= rg_kpm_moments' mu_out).
+ * group_out: [n_groups][M] complex, row-major (PdosMoments::group layout,
+ * pdos.hpp:129-141). With a partition of the rows, sum_g group = total to
+ * rounding. */
+RG_API int rg_kpm_pdos_moments(rg_context *ctx, int M, int precision, int doubling, int64_t n_rows,
+                        const int32_t *group_of_row, int n_groups, double *total_out,
+                        double *group_out);
+
 /* ---- adjoint (no reference counterpart: the differentiable form) ---- */
 
 /* Keeps the forward states of the next doubled FP64 rg_kpm_moments call for
@@ -216,7 +232,8 @@ RG_API int rg_set_profiling(rg_context *ctx, int on);
 #define RG_KCLASS_SPMM_PLAIN 0 /* BSR SpMM, Y = M X (sandwich inner factors) */
 #define RG_KCLASS_SPMM_STEP 1  /* SpMM fused with the recurrence and dot partials */
 #define RG_KCLASS_REDUCE 2     /* dot-partial reduction */
-#define RG_KCLASS_COUNT 3
+#define RG_KCLASS_GROUP 3      /* group-resolved dots of the projected moments */
+#define RG_KCLASS_COUNT 4
 
 /* Average launch duration (ms), launch count, and the algorithmic bytes and
  * flops of one launch of that class as last launched. */

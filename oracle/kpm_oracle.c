@@ -27,6 +27,8 @@
  *   or_kpm_moments (fused = 0) ......... kpm_dos_moments multivector
  *                                        (rsatb/calc/dos.hpp:124-189), the
  *                                        doubled and the plain recurrences
+ *   or_kpm_pdos_moments ................ kpm_pdos_moments multivector
+ *                                        (rsatb/calc/pdos.hpp:149-266)
  *   or_jackson_kernel .................. rsatb/calc/kpm.hpp:18-26
  *   or_kpm_kernel_weights .............. rsatb/calc/kpm.hpp:52-85
  *   or_kpm_density ..................... rsatb/calc/kpm.hpp:206-241
@@ -375,6 +377,96 @@ int or_kpm_moments(const or_op *op, const double *psi, int32_t R, int32_t M, int
     }
 #undef STORE_COLS
     free(t0); free(t1); free(t2); free(tmp); free(dots); free(dots2); free(c0); free(c1);
+    return 0;
+}
+
+/* ------------------------------------------------- projected moments --- */
+
+/* pdos.hpp:185-202: raw[slot] += sum_c conj(a(row, c)) b(row, c), slot 0 the
+ * all-rows channel, slot 1 + g for rows of group g >= 0. Serial over rows. */
+static void seg_dot(int64_t rows, int32_t R, const double *a, const double *b,
+                    const int32_t *grp, double *raw) {
+    for (int64_t i = 0; i < rows; ++i) {
+        double sr = 0.0, si = 0.0;
+        const double *ar = a + 2 * i * R, *brw = b + 2 * i * R;
+        for (int32_t c = 0; c < R; ++c) {
+            sr += ar[2 * c] * brw[2 * c] + ar[2 * c + 1] * brw[2 * c + 1];
+            si += ar[2 * c] * brw[2 * c + 1] - ar[2 * c + 1] * brw[2 * c];
+        }
+        raw[0] += sr;
+        raw[1] += si;
+        if (grp[i] >= 0) {
+            raw[2 * (1 + (int64_t)grp[i])] += sr;
+            raw[2 * (1 + (int64_t)grp[i]) + 1] += si;
+        }
+    }
+}
+
+/* kpm_pdos_moments multivector (pdos.hpp:149-266): the unfused recurrence of
+ * or_kpm_moments with the segmented dots, one reduction at the end, the
+ * division by R, then the per-channel doubling transform (:236-263).
+ *   group_of_row: [rows], ids < G or negative (no group).
+ *   total: [M] complex. group: [G][M] complex.
+ * Returns 0; -1 on bad arguments; -2 when a group id is >= G (:163-167). */
+int or_kpm_pdos_moments(const or_op *op, const double *psi, int32_t R, int32_t M,
+                        int32_t doubling, int32_t G, const int32_t *group_of_row, double *total,
+                        double *group) {
+    if (M < 1 || R < 1 || G < 0) return -1;
+    const int64_t rows = op->H->nbr * op->H->b;
+    for (int64_t i = 0; i < rows; ++i)
+        if (group_of_row[i] >= G) return -2;
+    const int64_t n = rows * (int64_t)R;
+    const int64_t width = 1 + (int64_t)G;
+    double *raw = (double *)calloc(2 * (size_t)M * (size_t)width, sizeof(double));
+    double *t0 = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *t1 = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *t2 = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    if (!raw || !t0 || !t1 || !t2 || !tmp) return -1;
+#define RAW(m) (raw + 2 * (size_t)(m) * (size_t)width)
+    memcpy(t0, psi, sizeof(double) * 2 * (size_t)n);
+    seg_dot(rows, R, t0, t0, group_of_row, RAW(0));
+    if (M > 1) {
+        or_apply_op(op, t0, R, t1, tmp);
+        seg_dot(rows, R, t0, t1, group_of_row, RAW(1));
+        if (doubling) {
+            for (int n_ = 1;; ++n_) {
+                if (2 * n_ - 1 < M && 2 * n_ - 1 > 1) seg_dot(rows, R, t1, t0, group_of_row, RAW(2 * n_ - 1));
+                if (2 * n_ < M) seg_dot(rows, R, t1, t1, group_of_row, RAW(2 * n_));
+                if (2 * n_ + 1 >= M) break;
+                or_apply_op(op, t1, R, t2, tmp);
+                v_axpby(n, -1.0, t0, 2.0, t2);
+                double *sw = t0; t0 = t1; t1 = t2; t2 = sw;
+            }
+        } else {
+            for (int32_t m = 2; m < M; ++m) {
+                or_apply_op(op, t1, R, t2, tmp);
+                v_axpby(n, -1.0, t0, 2.0, t2);
+                seg_dot(rows, R, psi, t2, group_of_row, RAW(m));
+                double *sw = t0; t0 = t1; t1 = t2; t2 = sw;
+            }
+        }
+    }
+    const double inv = 1.0 / (double)R;
+    for (size_t k = 0; k < 2 * (size_t)M * (size_t)width; ++k) raw[k] *= inv;
+    for (int64_t ch = 0; ch < width; ++ch) {
+        double *out = ch == 0 ? total : group + 2 * (size_t)(ch - 1) * (size_t)M;
+        const double *r0 = RAW(0) + 2 * ch;
+        const double *r1 = M > 1 ? RAW(1) + 2 * ch : NULL;
+        for (int32_t m = 0; m < M; ++m) {
+            const double *rm = RAW(m) + 2 * ch;
+            double re = rm[0], im = rm[1];
+            if (doubling && m >= 2) {
+                const double *base = (m % 2 == 0) ? r0 : r1;
+                re = 2.0 * re - base[0];
+                im = 2.0 * im - base[1];
+            }
+            out[2 * m] = re;
+            out[2 * m + 1] = im;
+        }
+    }
+#undef RAW
+    free(raw); free(t0); free(t1); free(t2); free(tmp);
     return 0;
 }
 

@@ -13,6 +13,8 @@ Dense references (numpy LAPACK, our own matrices):
     of A^{-1/2} belonging to block i, with f(x) = x^{-1/2} for x > 1e-12 else
     0 -- the clamped eigendecomposition route of rsatb/calc/isqrtm.hpp:167-173
     over the subgraph problems of graph_inverse_sqrt (isqrtm.hpp:137-186).
+  - dense_group_moments: sum over a group's rows of diag T_m(A) by the dense
+    recurrence, the check rsatb/calc/test/test_pdos.cpp:196-229 makes.
   - dos_grid / compute_dos: ComputeDos's kpm branch
     (rsatb/calc/observables.hpp:336-350, 370-393, 439-443).
 """
@@ -81,6 +83,9 @@ def load() -> C.CDLL:
     lib.or_apply_op.restype = C.c_int
     lib.or_kpm_moments.argtypes = [C.POINTER(_Op), pd, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pd, pd]
     lib.or_kpm_moments.restype = C.c_int
+    lib.or_kpm_pdos_moments.argtypes = [C.POINTER(_Op), pd, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                        C.POINTER(C.c_int32), pd, pd]
+    lib.or_kpm_pdos_moments.restype = C.c_int
     lib.or_jackson_kernel.argtypes = [C.c_int32, pd]
     lib.or_kpm_kernel_weights.argtypes = [C.c_int32, C.c_int32, pd]
     lib.or_kpm_kernel_weights.restype = C.c_int
@@ -187,6 +192,24 @@ class Operator:
             raise RuntimeError("or_kpm_moments failed")
         return (mu, cols) if per_column else mu
 
+    def pdos_moments(self, psi, M: int, group_of_row, n_groups: int, doubling: bool = True):
+        """kpm_pdos_moments: (total [M], group [n_groups, M]) complex."""
+        pc = _cvec(psi)
+        R = pc.shape[1]
+        grp = np.ascontiguousarray(np.asarray(group_of_row, dtype=np.int32))
+        if grp.size != self.rows:
+            raise RuntimeError("pdos: group_of_row must map every scalar row")
+        total = np.zeros(M, dtype=np.complex128)
+        group = np.zeros((n_groups, M), dtype=np.complex128)
+        rc = load().or_kpm_pdos_moments(C.byref(self.op), _pd(pc.view(np.float64)), R, M, int(doubling),
+                                        n_groups, grp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        _pd(total.view(np.float64)), _pd(group.view(np.float64)))
+        if rc == -2:
+            raise RuntimeError("pdos: group id exceeds n_groups")
+        if rc != 0:
+            raise RuntimeError("or_kpm_pdos_moments failed")
+        return total, group
+
 
 # ---------------------------------------------------------------- KPM
 def jackson_kernel(n: int) -> np.ndarray:
@@ -254,6 +277,31 @@ def dense_scaled(form: int, H, center: float, half_width: float, Z=None) -> np.n
         A = Hd
     n = A.shape[0]
     return (A - center * np.eye(n)) / half_width
+
+
+def dense_group_moments(A: np.ndarray, group_of_row, n_groups: int, M: int) -> np.ndarray:
+    """ref[g, m] = sum_{i in g} [T_m(A)]_ii by the plain three-term recurrence
+    on dense matrices (the check of rsatb/calc/test/test_pdos.cpp:196-229);
+    row n_groups is the all-rows channel."""
+    n = A.shape[0]
+    grp = np.asarray(group_of_row)
+    ref = np.zeros((n_groups + 1, M))
+    T0, T1 = np.eye(n, dtype=A.dtype), A.copy()
+
+    def record(m, T):
+        d = np.real(np.diag(T))
+        ref[n_groups, m] = d.sum()
+        mask = grp >= 0
+        np.add.at(ref[:, m], grp[mask], d[mask])
+
+    record(0, T0)
+    if M > 1:
+        record(1, T1)
+    for m in range(2, M):
+        T2 = 2.0 * (A @ T1) - T0
+        record(m, T2)
+        T0, T1 = T1, T2
+    return ref
 
 
 def dense_moments(A: np.ndarray, psi, M: int) -> np.ndarray:

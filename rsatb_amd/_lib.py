@@ -28,8 +28,9 @@ SAMPLES_RADEMACHER, SAMPLES_PHASE, SAMPLES_UPLOAD_REAL, SAMPLES_UPLOAD_COMPLEX =
 PREC_FP64, PREC_FP32 = 0, 1
 DOUBLING_AUTO, DOUBLING_OFF, DOUBLING_ON = -1, 0, 1
 KERNEL_NONE, KERNEL_JACKSON, KERNEL_COSTAPER, KERNEL_VPTAPER = 0, 1, 2, 3
-KCLASS_SPMM_PLAIN, KCLASS_SPMM_STEP, KCLASS_REDUCE = 0, 1, 2
-KCLASS_NAMES = {KCLASS_SPMM_PLAIN: "spmm_plain", KCLASS_SPMM_STEP: "spmm_step", KCLASS_REDUCE: "reduce"}
+KCLASS_SPMM_PLAIN, KCLASS_SPMM_STEP, KCLASS_REDUCE, KCLASS_GROUP = 0, 1, 2, 3
+KCLASS_NAMES = {KCLASS_SPMM_PLAIN: "spmm_plain", KCLASS_SPMM_STEP: "spmm_step", KCLASS_REDUCE: "reduce",
+                KCLASS_GROUP: "group_dots"}
 
 # Every symbol include/rsatb_gpu.h declares, with (restype, argtypes).
 _vp, _i, _i64, _d, _u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64
@@ -48,6 +49,7 @@ SIGNATURES = {
     "rg_apply_op": (_i, [_vp, _i, _pd, _pd, _i]),
     "rg_bsr_mult": (_i, [_vp, _i, _i, _pd, _pd, _i]),
     "rg_kpm_moments": (_i, [_vp, _i, _i, _i, _pd, _pd]),
+    "rg_kpm_pdos_moments": (_i, [_vp, _i, _i, _i, _i64, _pi32, _i, _pd, _pd]),
     "rg_dos_plan": (_i, [_d, _d, _d, C.POINTER(_i), C.POINTER(_i)]),
     "rg_compute_dos": (_i, [_vp, _d, _d, _d, _i, _pd, _pd]),
     "rg_kpm_density": (_i, [_pd, _i, _pd, _i64, _d, _i, _pd]),
@@ -271,6 +273,20 @@ class Context:
         )
         _check(self._h, rc, "rg_kpm_moments")
         return (mu, cols) if per_column else mu
+
+    def kpm_pdos_moments(self, n_moments: int, group_of_row, n_groups: int, precision: int = PREC_FP64,
+                         doubling: int = DOUBLING_AUTO):
+        """kpm_pdos_moments (rsatb/calc/pdos.hpp:149-266): (total [M], group [n_groups, M])."""
+        grp = np.ascontiguousarray(np.asarray(group_of_row, dtype=np.int32).reshape(-1))
+        total = np.zeros(n_moments, dtype=np.complex128)
+        group = np.zeros((max(int(n_groups), 0), n_moments), dtype=np.complex128)
+        rc = self._lib.rg_kpm_pdos_moments(
+            self._h, int(n_moments), int(precision), int(doubling), int(grp.size),
+            grp.ctypes.data_as(_pi32), int(n_groups), _dptr(total.view(np.float64)),
+            _dptr(group.view(np.float64)) if group.size else None,
+        )
+        _check(self._h, rc, "rg_kpm_pdos_moments")
+        return total, group
 
     def set_checkpointing(self, stride: int) -> None:
         _check(self._h, self._lib.rg_set_checkpointing(self._h, int(stride)), "rg_set_checkpointing")

@@ -17,9 +17,10 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librsatb_gpu.so"
 
-SOURCES = ["rg_kernels.hip", "rg_isqrt.hip", "rg_lanczos.hip", "rg_adjoint.hip", "rg_engine.hip"]
+SOURCES = ["rg_kernels.hip", "rg_isqrt.hip", "rg_lanczos.hip", "rg_adjoint.hip", "rg_pdos.hip",
+           "rg_engine.hip"]
 
-# -packed-fp32-ops: no v_pk_* FP32 arithmetic is generated.
+# -packed-fp32-ops: hipcc generates no packed (v_pk_*) FP32 arithmetic.
 HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",

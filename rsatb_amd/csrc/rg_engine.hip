@@ -92,6 +92,8 @@ struct rg_context {
 
     DevBuf vec[6]; // t0, t1, t2, tmp1, tmp2, psi copy
     DevBuf partials, raw;
+    // projected moments: rows sorted by group, chunk and group ranges, dots
+    DevBuf pd_rows, pd_chunk, pd_grp, pd_partial, pd_out;
 
     std::atomic<int> cancel{0};
 
@@ -277,18 +279,57 @@ int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out) {
     return RG_OK;
 }
 
+// Group-resolved dots of one step (projected moments): rows sorted by group,
+// cut into chunks; out[d * G + g] for d < ndot.
+struct GroupDots {
+    int64_t G = 0, nchunks = 0;
+    const int64_t *chunk_ptr = nullptr; // [nchunks + 1] into rows
+    const int32_t *rows = nullptr;      // rows of group 0, then group 1, ...
+    const int64_t *grp_chunk = nullptr; // [G + 1] into chunks
+    double2 *partial = nullptr;         // [nchunks][3]
+    double2 *out = nullptr;             // [steps][3][G]
+    double bytes = 0.0;                 // algorithmic bytes of one launch pair
+};
+
+int group_dots(rg_context *ctx, const GroupDots &gd, VecMode vm, int R, int ndot, const void *v0,
+               const void *v1, int step) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (!a || !b) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
+        RG_HIP(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
+    }
+    hipError_t he = launch_group_dots(gd.nchunks, gd.chunk_ptr, gd.rows, R, ndot, vm, v0, v1,
+                                      gd.partial, ctx->stream);
+    if (he != hipSuccess) return hipfail(ctx, he, "group dot launch");
+    he = launch_group_finish(gd.G, gd.grp_chunk, ndot, gd.partial,
+                             gd.out + (size_t)step * 3 * gd.G, ctx->stream);
+    if (he != hipSuccess) return hipfail(ctx, he, "group dot launch");
+    if (ctx->prof) {
+        RG_HIP(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
+        ctx->pending.push_back({a, b, RG_KCLASS_GROUP});
+        ctx->stats[RG_KCLASS_GROUP].bytes = gd.bytes;
+        ctx->stats[RG_KCLASS_GROUP].flops = 0.0;
+    }
+    return RG_OK;
+}
+
 // y = chain(x) with the epilogue of the last factor (Chebyshev::apply_form,
 // cheby.hpp:555-591): Orthogonal H x; Sandwich Z^H (H (Z x)).
+// zx (sandwich only, optional): where Z x is written instead of the scratch
+// vector, for callers that reuse it.
 int apply_chain(rg_context *ctx, VecMode vm, const void *x, void *y, int R, EpiArgs e,
-                int *grid_out, int form = -1) {
+                int *grid_out, int form = -1, void *zx = nullptr) {
     if (form < 0) form = ctx->form;
     e.xs = x;
     if (form == RG_FORM_ORTHOGONAL) return spmm(ctx, RG_SLOT_H, vm, x, y, R, e, grid_out);
     EpiArgs plain;
     plain.mode = EPI_PLAIN;
-    int rc = spmm(ctx, RG_SLOT_Z, vm, x, ctx->vec[3].p, R, plain, nullptr);
+    void *t1 = zx ? zx : ctx->vec[3].p;
+    int rc = spmm(ctx, RG_SLOT_Z, vm, x, t1, R, plain, nullptr);
     if (rc) return rc;
-    rc = spmm(ctx, RG_SLOT_H, vm, ctx->vec[3].p, ctx->vec[4].p, R, plain, nullptr);
+    rc = spmm(ctx, RG_SLOT_H, vm, t1, ctx->vec[4].p, R, plain, nullptr);
     if (rc) return rc;
     return spmm(ctx, RG_SLOT_ZH, vm, ctx->vec[4].p, y, R, e, grid_out);
 }
@@ -375,6 +416,207 @@ int64_t op_rows(const rg_context *ctx) {
     return ctx->slots[RG_SLOT_H].nbr * (int64_t)ctx->slots[RG_SLOT_H].b;
 }
 
+// Moments from the dots each step stored (dos.hpp:150-187, pdos.hpp:240-258):
+// step 0 holds <t0|t0>, <t0|t1>, <t1|t1>; a doubled step n >= 1 holds
+// <psi_{n+1}|psi_{n+1}>, <psi_{n+1}|psi_n>; a plain step n holds <psi|psi_{n+1}>.
+template <class Get, class Put>
+void moments_from_dots(int M, int nraw, bool doubled, Get get, Put put) {
+    const double2 d0 = get(0, 0), d1 = get(0, 1), d2 = get(0, 2);
+    put(0, d0);
+    if (M > 1) put(1, d1);
+    if (doubled) {
+        if (M > 2) put(2, double2{2.0 * d2.x - d0.x, 2.0 * d2.y - d0.y});
+        for (int n = 1; n < nraw; ++n) {
+            const double2 e0 = get(n, 0), e1 = get(n, 1);
+            const int mo = 2 * n + 1, me = 2 * n + 2;
+            if (mo < M) put(mo, double2{2.0 * e1.x - d1.x, 2.0 * e1.y - d1.y});
+            if (me < M) put(me, double2{2.0 * e0.x - d0.x, 2.0 * e0.y - d0.y});
+        }
+    } else {
+        for (int n = 1; n < nraw; ++n) put(n + 1, get(n, 0));
+    }
+}
+
+// The moment loop of rg_kpm_moments; with gd, the group-resolved dots of
+// every step are written to gd->out as well (steps as moments_from_dots reads
+// them; nraw_out receives the step count).
+int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
+            double *mu_cols_out, const GroupDots *gd, int *nraw_out = nullptr) {
+    int rc = check_operator(ctx);
+    if (rc) return rc;
+    if (!ctx->samples_set) return fail(ctx, RG_ERR_STATE, "samples not set (rg_set_samples)");
+    if (M < 1) return fail(ctx, RG_ERR_INVALID, "n_moments must be >= 1");
+    if (precision != RG_PREC_FP64 && precision != RG_PREC_FP32)
+        return fail(ctx, RG_ERR_INVALID, "unknown precision");
+    const int64_t rows = op_rows(ctx);
+    if (ctx->n_rows != rows)
+        return fail(ctx, RG_ERR_INVALID,
+                    "sample rows (" + std::to_string(ctx->n_rows) + ") differ from operator rows (" +
+                        std::to_string(rows) + ")");
+    const bool doubled = (doubling != RG_DOUBLING_OFF); // both forms are Hermitian
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->cancel.store(0);
+
+    const int R = ctx->R;
+    VecMode vm;
+    vm.complex_vec = vec_complex(ctx);
+    vm.fp32 = (precision == RG_PREC_FP32);
+    if (vm.fp32) {
+        rc = ensure_fp32(ctx);
+        if (rc) return rc;
+    }
+    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
+    for (int i = 0; i < 6; ++i) {
+        if ((i == 3 || i == 4) && ctx->form != RG_FORM_SANDWICH) continue;
+        if (i == 5 && doubled) continue;
+        RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
+    }
+    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
+    const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
+    RG_HIP(ctx, ctx->raw.ensure((size_t)nsteps * 3 * R * 16), "hipMalloc moment store");
+    double2 *raw = static_cast<double2 *>(ctx->raw.p);
+
+    // forward states kept for the adjoint: psi_j for j <= 1 and j = 0, K-1 (mod K)
+    const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32;
+    ctx->ckpt_valid = false;
+    int Npsi = 1;
+    for (int n = 1; 2 * n + 1 < M; ++n) ++Npsi;
+    if (keep) {
+        const int K = ctx->ckpt_stride;
+        ctx->ckpt_slot.assign((size_t)Npsi + 1, -1);
+        int nslot = 0;
+        for (int j = 0; j <= Npsi; ++j)
+            if (j <= 1 || K == 1 || j % K == 0 || j % K == K - 1) ctx->ckpt_slot[j] = nslot++;
+        RG_HIP(ctx, ctx->ckpt.ensure((size_t)nslot * nb), "hipMalloc forward checkpoints");
+    }
+    auto kept = [&](int j) -> void * {
+        if (!keep || ctx->ckpt_slot[j] < 0) return nullptr;
+        return static_cast<char *>(ctx->ckpt.p) + (size_t)ctx->ckpt_slot[j] * nb;
+    };
+    // a work buffer that holds neither live state
+    auto scratch = [&](const void *x0, const void *x1) -> void * {
+        for (int i = 0; i < 3; ++i)
+            if (ctx->vec[i].p != x0 && ctx->vec[i].p != x1) return ctx->vec[i].p;
+        return nullptr;
+    };
+
+    void *t0 = kept(0) ? kept(0) : ctx->vec[0].p;
+    void *t1 = kept(1) ? kept(1) : ctx->vec[1].p;
+    void *t2 = ctx->vec[2].p;
+    rc = stage_samples(ctx, vm, t0);
+    if (rc) return rc;
+    if (!doubled) {
+        RG_HIP(ctx, hipMemcpyAsync(ctx->vec[5].p, t0, nb, hipMemcpyDeviceToDevice, ctx->stream),
+               "psi copy");
+    }
+
+    const double a = ctx->half_width, c = ctx->center;
+    EpiArgs e;
+    e.partials = static_cast<double2 *>(ctx->partials.p);
+
+    // t1 = A~ t0 with <t0|t0>, <t0|t1>, <t1|t1> (dos.hpp:141-148, 160-163 at n = 1)
+    e.mode = EPI_FIRST;
+    e.alpha = 1.0 / a;
+    e.beta = -c / a;
+    int g = 0;
+    rc = apply_chain(ctx, vm, t0, t1, R, e, &g);
+    if (rc) return rc;
+    rc = reduce(ctx, g, 3, R, raw);
+    if (rc) return rc;
+    if (gd) {
+        rc = group_dots(ctx, *gd, vm, R, 3, t0, t1, 0);
+        if (rc) return rc;
+    }
+
+    int nraw = 1;
+    e.alpha = 2.0 / a;
+    e.beta = -2.0 * c / a;
+    if (doubled) {
+        // t1 = psi_n, t0 = psi_{n-1}; t2 = psi_{n+1} with <t2|t2>, <t2|t1>
+        e.mode = EPI_STEP;
+        for (int n = 1; 2 * n + 1 < M; ++n) {
+            if (ctx->cancel.load()) {
+                (void)hipStreamSynchronize(ctx->stream);
+                drain_events(ctx);
+                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+            }
+            t2 = kept(n + 1) ? kept(n + 1) : scratch(t0, t1);
+            e.xp = t0;
+            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
+            if (rc) return rc;
+            rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R);
+            if (rc) return rc;
+            if (gd) {
+                rc = group_dots(ctx, *gd, vm, R, 2, t2, t1, nraw);
+                if (rc) return rc;
+            }
+            ++nraw;
+            t0 = t1;
+            t1 = t2;
+        }
+        if (keep) {
+            ctx->ckpt_valid = true;
+            ctx->ckpt_M = M;
+            ctx->ckpt_N = Npsi;
+            ctx->ckpt_R = R;
+            ctx->ckpt_complex = vm.complex_vec;
+        }
+    } else {
+        e.mode = EPI_PLAINSTEP;
+        e.psi = ctx->vec[5].p;
+        for (int m = 2; m < M; ++m) {
+            if (ctx->cancel.load()) {
+                (void)hipStreamSynchronize(ctx->stream);
+                drain_events(ctx);
+                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+            }
+            e.xp = t0;
+            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
+            if (rc) return rc;
+            rc = reduce(ctx, g, 1, R, raw + (size_t)nraw * 3 * R);
+            if (rc) return rc;
+            if (gd) {
+                rc = group_dots(ctx, *gd, vm, R, 1, ctx->vec[5].p, t2, nraw);
+                if (rc) return rc;
+            }
+            ++nraw;
+            void *sw = t0;
+            t0 = t1;
+            t1 = t2;
+            t2 = sw;
+        }
+    }
+    std::vector<double2> host((size_t)nraw * 3 * R);
+    RG_HIP(ctx, hipMemcpyAsync(host.data(), raw, host.size() * sizeof(double2),
+                               hipMemcpyDeviceToHost, ctx->stream),
+           "moment download");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->prof) drain_events(ctx);
+
+    // per-column moments, then the column mean (dos.hpp:133-138)
+    std::vector<double> mc((size_t)M * R * 2, 0.0);
+    for (int col = 0; col < R; ++col)
+        moments_from_dots(
+            M, nraw, doubled,
+            [&](int step, int d) { return host[((size_t)step * 3 + d) * R + col]; },
+            [&](int m, double2 v) {
+                mc[((size_t)m * R + col) * 2] = v.x;
+                mc[((size_t)m * R + col) * 2 + 1] = v.y;
+            });
+    for (int m = 0; m < M; ++m) {
+        double sr = 0.0, si = 0.0;
+        for (int col = 0; col < R; ++col) {
+            sr += mc[((size_t)m * R + col) * 2];
+            si += mc[((size_t)m * R + col) * 2 + 1];
+        }
+        mu_out[2 * m] = sr / R;
+        mu_out[2 * m + 1] = si / R;
+    }
+    if (mu_cols_out) std::memcpy(mu_cols_out, mc.data(), mc.size() * sizeof(double));
+    if (nraw_out) *nraw_out = nraw;
+    return RG_OK;
+}
+
 } // namespace
 
 // ===================================================================== ABI
@@ -392,9 +634,9 @@ int rg_create(int device_id, rg_context **out) {
     rg_context *ctx = new rg_context();
     ctx->device = device_id;
     if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
-    if (const char *v = std::getenv("RG_MFMA_PREFETCH")) set_mfma_prefetch(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
     if (const char *v = std::getenv("RG_UNION")) set_union_stage(std::atoi(v));
+    if (const char *v = std::getenv("RG_EPU")) set_epi_unroll(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RG_ERR_HIP;
@@ -426,6 +668,8 @@ void rg_destroy(rg_context *ctx) {
     for (auto &v : ctx->vec) v.release();
     ctx->partials.release();
     ctx->raw.release();
+    for (DevBuf *b : {&ctx->pd_rows, &ctx->pd_chunk, &ctx->pd_grp, &ctx->pd_partial, &ctx->pd_out})
+        b->release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -588,181 +832,100 @@ int rg_apply_op(rg_context *ctx, int R, const double *x, double *y, int is_compl
 int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
                    double *mu_cols_out) {
     if (!ctx || !mu_out) return RG_ERR_INVALID;
-    int rc = check_operator(ctx);
-    if (rc) return rc;
+    return kpm_run(ctx, M, precision, doubling, mu_out, mu_cols_out, nullptr);
+}
+
+int rg_kpm_pdos_moments(rg_context *ctx, int M, int precision, int doubling, int64_t n_rows,
+                        const int32_t *group_of_row, int n_groups, double *total_out,
+                        double *group_out) {
+    if (!ctx || !total_out) return RG_ERR_INVALID;
+    if (n_groups < 0) return fail(ctx, RG_ERR_INVALID, "pdos: n_groups must be >= 0");
+    if ((n_groups > 0 && !group_out) || (n_rows > 0 && !group_of_row))
+        return fail(ctx, RG_ERR_INVALID, "pdos: null group array");
     if (!ctx->samples_set) return fail(ctx, RG_ERR_STATE, "samples not set (rg_set_samples)");
     if (M < 1) return fail(ctx, RG_ERR_INVALID, "n_moments must be >= 1");
-    if (precision != RG_PREC_FP64 && precision != RG_PREC_FP32)
-        return fail(ctx, RG_ERR_INVALID, "unknown precision");
-    const int64_t rows = op_rows(ctx);
-    if (ctx->n_rows != rows)
+    // pdos.hpp:159-167
+    if (n_rows != ctx->n_rows)
         return fail(ctx, RG_ERR_INVALID,
-                    "sample rows (" + std::to_string(ctx->n_rows) + ") differ from operator rows (" +
-                        std::to_string(rows) + ")");
-    const bool doubled = (doubling != RG_DOUBLING_OFF); // both forms are Hermitian
-    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    ctx->cancel.store(0);
-
+                    "pdos: group_of_row must map every scalar row (got " + std::to_string(n_rows) +
+                        " for " + std::to_string(ctx->n_rows) + " rows)");
+    if (n_rows > INT32_MAX) return fail(ctx, RG_ERR_UNSUPPORTED, "pdos: more than 2^31 rows");
+    const int64_t G = n_groups;
+    std::vector<int64_t> start((size_t)G + 1, 0);
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const int32_t g = group_of_row[r];
+        if (g >= n_groups) return fail(ctx, RG_ERR_INVALID, "pdos: group id exceeds n_groups");
+        if (g >= 0) ++start[(size_t)g + 1];
+    }
+    for (int64_t g = 0; g < G; ++g) start[g + 1] += start[g];
+    const int64_t ngrouped = start[G];
+    std::vector<int32_t> rows((size_t)std::max<int64_t>(ngrouped, 1));
+    {
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t r = 0; r < n_rows; ++r)
+            if (group_of_row[r] >= 0) rows[(size_t)fill[group_of_row[r]]++] = (int32_t)r;
+    }
+    // chunks of about 4096 (row, column) elements, never spanning two groups
     const int R = ctx->R;
+    const int64_t rpc = std::max<int64_t>(1, 4096 / R);
+    std::vector<int64_t> chunk_ptr, grp_chunk((size_t)G + 1, 0);
+    for (int64_t g = 0; g < G; ++g) {
+        for (int64_t p = start[g]; p < start[g + 1]; p += rpc) chunk_ptr.push_back(p);
+        grp_chunk[g + 1] = (int64_t)chunk_ptr.size();
+    }
+    const int64_t nchunks = (int64_t)chunk_ptr.size();
+    chunk_ptr.push_back(ngrouped);
+
+    const bool doubled = (doubling != RG_DOUBLING_OFF);
+    const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    RG_HIP(ctx, ctx->pd_rows.ensure(rows.size() * 4), "hipMalloc group rows");
+    RG_HIP(ctx, ctx->pd_chunk.ensure(chunk_ptr.size() * 8), "hipMalloc group chunks");
+    RG_HIP(ctx, ctx->pd_grp.ensure(grp_chunk.size() * 8), "hipMalloc group chunks");
+    RG_HIP(ctx, ctx->pd_partial.ensure((size_t)std::max<int64_t>(nchunks, 1) * 3 * 16),
+           "hipMalloc group partials");
+    RG_HIP(ctx, ctx->pd_out.ensure((size_t)nsteps * 3 * std::max<int64_t>(G, 1) * 16),
+           "hipMalloc group dots");
+    RG_HIP(ctx, hipMemcpy(ctx->pd_rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice),
+           "upload group rows");
+    RG_HIP(ctx, hipMemcpy(ctx->pd_chunk.p, chunk_ptr.data(), chunk_ptr.size() * 8,
+                          hipMemcpyHostToDevice),
+           "upload group chunks");
+    RG_HIP(ctx, hipMemcpy(ctx->pd_grp.p, grp_chunk.data(), grp_chunk.size() * 8,
+                          hipMemcpyHostToDevice),
+           "upload group chunks");
+
+    GroupDots gd;
+    gd.G = G;
+    gd.nchunks = nchunks;
+    gd.chunk_ptr = static_cast<const int64_t *>(ctx->pd_chunk.p);
+    gd.rows = static_cast<const int32_t *>(ctx->pd_rows.p);
+    gd.grp_chunk = static_cast<const int64_t *>(ctx->pd_grp.p);
+    gd.partial = static_cast<double2 *>(ctx->pd_partial.p);
+    gd.out = static_cast<double2 *>(ctx->pd_out.p);
     VecMode vm;
     vm.complex_vec = vec_complex(ctx);
     vm.fp32 = (precision == RG_PREC_FP32);
-    if (vm.fp32) {
-        rc = ensure_fp32(ctx);
-        if (rc) return rc;
-    }
-    const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
-    for (int i = 0; i < 6; ++i) {
-        if ((i == 3 || i == 4) && ctx->form != RG_FORM_SANDWICH) continue;
-        if (i == 5 && doubled) continue;
-        RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
-    }
-    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
-    const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
-    RG_HIP(ctx, ctx->raw.ensure((size_t)nsteps * 3 * R * 16), "hipMalloc moment store");
-    double2 *raw = static_cast<double2 *>(ctx->raw.p);
+    gd.bytes = 2.0 * (double)ngrouped * R * (double)vec_elem_bytes(vm);
 
-    // forward states kept for the adjoint: psi_j for j <= 1 and j = 0, K-1 (mod K)
-    const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32;
-    ctx->ckpt_valid = false;
-    int Npsi = 1;
-    for (int n = 1; 2 * n + 1 < M; ++n) ++Npsi;
-    if (keep) {
-        const int K = ctx->ckpt_stride;
-        ctx->ckpt_slot.assign((size_t)Npsi + 1, -1);
-        int nslot = 0;
-        for (int j = 0; j <= Npsi; ++j)
-            if (j <= 1 || K == 1 || j % K == 0 || j % K == K - 1) ctx->ckpt_slot[j] = nslot++;
-        RG_HIP(ctx, ctx->ckpt.ensure((size_t)nslot * nb), "hipMalloc forward checkpoints");
-    }
-    auto kept = [&](int j) -> void * {
-        if (!keep || ctx->ckpt_slot[j] < 0) return nullptr;
-        return static_cast<char *>(ctx->ckpt.p) + (size_t)ctx->ckpt_slot[j] * nb;
-    };
-    // a work buffer that holds neither live state
-    auto scratch = [&](const void *x0, const void *x1) -> void * {
-        for (int i = 0; i < 3; ++i)
-            if (ctx->vec[i].p != x0 && ctx->vec[i].p != x1) return ctx->vec[i].p;
-        return nullptr;
-    };
-
-    void *t0 = kept(0) ? kept(0) : ctx->vec[0].p;
-    void *t1 = kept(1) ? kept(1) : ctx->vec[1].p;
-    void *t2 = ctx->vec[2].p;
-    rc = stage_samples(ctx, vm, t0);
+    int nraw = 0;
+    int rc = kpm_run(ctx, M, precision, doubling, total_out, nullptr, &gd, &nraw);
     if (rc) return rc;
-    if (!doubled) {
-        RG_HIP(ctx, hipMemcpyAsync(ctx->vec[5].p, t0, nb, hipMemcpyDeviceToDevice, ctx->stream),
-               "psi copy");
-    }
-
-    const double a = ctx->half_width, c = ctx->center;
-    EpiArgs e;
-    e.partials = static_cast<double2 *>(ctx->partials.p);
-
-    // t1 = A~ t0 with <t0|t0>, <t0|t1>, <t1|t1> (dos.hpp:141-148, 160-163 at n = 1)
-    e.mode = EPI_FIRST;
-    e.alpha = 1.0 / a;
-    e.beta = -c / a;
-    int g = 0;
-    rc = apply_chain(ctx, vm, t0, t1, R, e, &g);
-    if (rc) return rc;
-    rc = reduce(ctx, g, 3, R, raw);
-    if (rc) return rc;
-
-    int nraw = 1;
-    e.alpha = 2.0 / a;
-    e.beta = -2.0 * c / a;
-    if (doubled) {
-        // t1 = psi_n, t0 = psi_{n-1}; t2 = psi_{n+1} with <t2|t2>, <t2|t1>
-        e.mode = EPI_STEP;
-        for (int n = 1; 2 * n + 1 < M; ++n) {
-            if (ctx->cancel.load()) {
-                (void)hipStreamSynchronize(ctx->stream);
-                drain_events(ctx);
-                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
-            }
-            t2 = kept(n + 1) ? kept(n + 1) : scratch(t0, t1);
-            e.xp = t0;
-            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
-            if (rc) return rc;
-            rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R);
-            if (rc) return rc;
-            ++nraw;
-            t0 = t1;
-            t1 = t2;
-        }
-        if (keep) {
-            ctx->ckpt_valid = true;
-            ctx->ckpt_M = M;
-            ctx->ckpt_N = Npsi;
-            ctx->ckpt_R = R;
-            ctx->ckpt_complex = vm.complex_vec;
-        }
-    } else {
-        e.mode = EPI_PLAINSTEP;
-        e.psi = ctx->vec[5].p;
-        for (int m = 2; m < M; ++m) {
-            if (ctx->cancel.load()) {
-                (void)hipStreamSynchronize(ctx->stream);
-                drain_events(ctx);
-                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
-            }
-            e.xp = t0;
-            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
-            if (rc) return rc;
-            rc = reduce(ctx, g, 1, R, raw + (size_t)nraw * 3 * R);
-            if (rc) return rc;
-            ++nraw;
-            void *sw = t0;
-            t0 = t1;
-            t1 = t2;
-            t2 = sw;
-        }
-    }
-    std::vector<double2> host((size_t)nraw * 3 * R);
-    RG_HIP(ctx, hipMemcpyAsync(host.data(), raw, host.size() * sizeof(double2),
-                               hipMemcpyDeviceToHost, ctx->stream),
-           "moment download");
-    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    if (ctx->prof) drain_events(ctx);
-
-    // per-column moments, then the column mean (dos.hpp:133-138)
-    std::vector<double> mc((size_t)M * R * 2, 0.0);
-    auto put = [&](int m, int col, double re, double im) {
-        mc[((size_t)m * R + col) * 2] = re;
-        mc[((size_t)m * R + col) * 2 + 1] = im;
-    };
-    auto rawv = [&](int step, int d, int col) { return host[((size_t)step * 3 + d) * R + col]; };
-    for (int col = 0; col < R; ++col) {
-        const double2 d0 = rawv(0, 0, col), d1 = rawv(0, 1, col), d2 = rawv(0, 2, col);
-        put(0, col, d0.x, d0.y);
-        if (M > 1) put(1, col, d1.x, d1.y);
-        if (doubled) {
-            if (M > 2) put(2, col, 2.0 * d2.x - d0.x, 2.0 * d2.y - d0.y);
-            for (int n = 1; n < nraw; ++n) {
-                const double2 e0 = rawv(n, 0, col), e1 = rawv(n, 1, col);
-                const int mo = 2 * n + 1, me = 2 * n + 2;
-                if (mo < M) put(mo, col, 2.0 * e1.x - d1.x, 2.0 * e1.y - d1.y);
-                if (me < M) put(me, col, 2.0 * e0.x - d0.x, 2.0 * e0.y - d0.y);
-            }
-        } else {
-            for (int n = 1; n < nraw; ++n) {
-                const double2 e0 = rawv(n, 0, col);
-                put(n + 1, col, e0.x, e0.y);
-            }
-        }
-    }
-    for (int m = 0; m < M; ++m) {
-        double sr = 0.0, si = 0.0;
-        for (int col = 0; col < R; ++col) {
-            sr += mc[((size_t)m * R + col) * 2];
-            si += mc[((size_t)m * R + col) * 2 + 1];
-        }
-        mu_out[2 * m] = sr / R;
-        mu_out[2 * m + 1] = si / R;
-    }
-    if (mu_cols_out) std::memcpy(mu_cols_out, mc.data(), mc.size() * sizeof(double));
+    if (G == 0) return RG_OK;
+    std::vector<double2> host((size_t)nraw * 3 * G);
+    RG_HIP(ctx, hipMemcpy(host.data(), ctx->pd_out.p, host.size() * sizeof(double2),
+                          hipMemcpyDeviceToHost),
+           "group moment download");
+    // doubling per group, then the column mean (pdos.hpp:236-263)
+    const double inv = 1.0 / (double)R;
+    for (int64_t g = 0; g < G; ++g)
+        moments_from_dots(
+            M, nraw, doubled,
+            [&](int step, int d) { return host[((size_t)step * 3 + d) * G + g]; },
+            [&](int m, double2 v) {
+                group_out[2 * ((size_t)g * M + m)] = v.x * inv;
+                group_out[2 * ((size_t)g * M + m) + 1] = v.y * inv;
+            });
     return RG_OK;
 }
 
@@ -1084,13 +1247,11 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
 
     // buffers: 3 adjoint states, a zero vector, K + 2 recomputed forward
     // states, two sandwich transforms, the gradient on H's pattern
-    DevBuf lam[3], zero, seg, zl, zp, grad;
+    DevBuf lam[3], zero, seg, grad;
     for (auto &l : lam) RG_HIP(ctx, l.ensure(nb), "hipMalloc adjoint");
     RG_HIP(ctx, zero.ensure(nb), "hipMalloc");
     RG_HIP(ctx, seg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
     if (sandwich) {
-        RG_HIP(ctx, zl.ensure(nb), "hipMalloc");
-        RG_HIP(ctx, zp.ensure(nb), "hipMalloc");
         RG_HIP(ctx, ctx->vec[3].ensure(nb), "hipMalloc");
         RG_HIP(ctx, ctx->vec[4].ensure(nb), "hipMalloc");
     }
@@ -1107,13 +1268,32 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
     };
     // adjoint states: lam_{n+2}, lam_{n+1}, lam_n rotate through lam[]
     void *l2 = lam[0].p, *l1 = lam[1].p, *l0 = lam[2].p;
+    // sandwich: Z psi_j of the segment, mostly produced by the recompute
+    // chains themselves (their first factor), the rest by one Z product each
+    DevBuf zseg;
+    if (sandwich) RG_HIP(ctx, zseg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
+    EpiArgs plain;
+    plain.mode = EPI_PLAIN;
+    auto accum = [&](const void *u, const void *v, double coef) -> int {
+        RG_HIP(ctx, launch_grad_accum(Hs.nbr, Hs.b, Hs.nnzb, (const int64_t *)Hs.row_ptr.p,
+                                      (const int32_t *)Hs.col_ind.p, (const double *)u,
+                                      (const double *)v, R, vm.complex_vec, coef,
+                                      (double *)grad.p, ctx->stream),
+               "gradient accumulation");
+        return RG_OK;
+    };
     const int nseg = N / K;
+    void *zpsi0 = nullptr;
     for (int k = nseg; k >= 0; --k) {
         const int lo = std::max(1, k * K), hi = std::min(k * K + K - 1, N);
         if (lo > hi) continue;
         // forward states psi_{lo-1} .. psi_{min(hi+1, N)}: kept or recomputed
         const int jlo = lo - 1, jhi = std::min(hi + 1, N);
         std::vector<void *> ptr((size_t)(jhi - jlo + 1), nullptr);
+        std::vector<bool> zdone((size_t)(jhi - jlo + 1), false);
+        auto zp_of = [&](int j) -> void * {
+            return static_cast<char *>(zseg.p) + (size_t)(j - jlo) * nb;
+        };
         int next_seg = 0;
         EpiArgs ef;
         ef.mode = EPI_STEP; // its dots land in the partial buffer and are not read
@@ -1126,10 +1306,21 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
                 if (j < 2 || j - 2 < jlo) return fail(ctx, RG_ERR_STATE, "checkpoint layout broken");
                 p = static_cast<char *>(seg.p) + (size_t)(next_seg++) * nb;
                 ef.xp = ptr[j - 2 - jlo];
-                rc = apply_chain(ctx, vm, ptr[j - 1 - jlo], p, R, ef, nullptr);
+                rc = apply_chain(ctx, vm, ptr[j - 1 - jlo], p, R, ef, nullptr, -1,
+                                 sandwich ? zp_of(j - 1) : nullptr);
                 if (rc) return rc;
+                if (sandwich) zdone[j - 1 - jlo] = true;
             }
             ptr[j - jlo] = p;
+        }
+        if (sandwich) {
+            for (int j = (k == 0 ? 0 : lo); j <= hi; ++j) {
+                if (zdone[j - jlo]) continue;
+                rc = spmm(ctx, RG_SLOT_Z, vm, ptr[j - jlo], zp_of(j), R, plain, nullptr);
+                if (rc) return rc;
+                zdone[j - jlo] = true;
+            }
+            if (k == 0) zpsi0 = zp_of(0);
         }
         for (int n = hi; n >= lo; --n) {
             if (ctx->cancel.load()) {
@@ -1152,29 +1343,27 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
             // lam_n = 2 A~ lam_{n+1} - lam_{n+2} + d_n  (lam_{N+1} = lam_{N+2} = 0)
             rc = apply_chain(ctx, vm, l1, l0, R, e, nullptr);
             if (rc) return rc;
-            // dL/dA~ += coef lam_n psi_{n-1}^T (1 for psi_1 = A~ psi_0, 2 after)
-            const double coef = (n >= 2 ? 2.0 : 1.0) / a;
-            const void *u = l0, *v = ptr[n - 1 - jlo];
-            if (sandwich) { // dL/dH = Z (dL/dA) Z^H
-                EpiArgs pl;
-                pl.mode = EPI_PLAIN;
-                rc = spmm(ctx, RG_SLOT_Z, vm, l0, zl.p, R, pl, nullptr);
+            if (!sandwich) {
+                // dL/dA~ += coef lam_n psi_{n-1}^T (1 for psi_1 = A~ psi_0, 2 after)
+                rc = accum(l0, ptr[n - 1 - jlo], (n >= 2 ? 2.0 : 1.0) / a);
                 if (rc) return rc;
-                rc = spmm(ctx, RG_SLOT_Z, vm, v, zp.p, R, pl, nullptr);
+            } else if (has_next) {
+                // dL/dH = Z (dL/dA) Z^H: the chain just formed Z lam_{n+1}, the
+                // pair (lam_{n+1}, psi_n) enters with coefficient 2
+                rc = accum(ctx->vec[3].p, zp_of(n), 2.0 / a);
                 if (rc) return rc;
-                u = zl.p;
-                v = zp.p;
             }
-            RG_HIP(ctx, launch_grad_accum(Hs.nbr, Hs.b, Hs.nnzb, (const int64_t *)Hs.row_ptr.p,
-                                          (const int32_t *)Hs.col_ind.p, (const double *)u,
-                                          (const double *)v, R, vm.complex_vec, coef,
-                                          (double *)grad.p, ctx->stream),
-                   "gradient accumulation");
             void *old = l2;
             l2 = l1;
             l1 = l0;
             l0 = old;
         }
+    }
+    if (sandwich) { // the last pair, (lam_1, psi_0), coefficient 1
+        rc = spmm(ctx, RG_SLOT_Z, vm, l1, zero.p, R, plain, nullptr);
+        if (rc) return rc;
+        rc = accum(zero.p, zpsi0, 1.0 / a);
+        if (rc) return rc;
     }
     std::vector<double> cm((size_t)ng);
     RG_HIP(ctx, hipMemcpyAsync(cm.data(), grad.p, (size_t)ng * 8, hipMemcpyDeviceToHost, ctx->stream), "download");

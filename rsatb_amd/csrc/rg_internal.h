@@ -65,9 +65,9 @@ int spmm_grid(int64_t nbr, int R);
 int spmm_max_grid();
 // XCD-contiguous row ranges in the MFMA SpMM (default on).
 void set_xcd_map(int on);
-void set_mfma_prefetch(int on);
 void set_mfma_ntcap(int nt);
 void set_union_stage(int on);
+void set_epi_unroll(int n);
 int union_group_rows();
 
 // Device matrices hold every b x b block column-major; uploads arrive
@@ -95,6 +95,18 @@ hipError_t launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_
 // widening real to complex (zero imaginary part) and/or narrowing to FP32.
 hipError_t launch_stage(const double *in, bool in_complex, void *out, bool out_complex,
                         bool out_fp32, int64_t n, hipStream_t stream);
+
+// ---- projected moments (rg_pdos.hip) ----
+// partial[c][3] for chunk c = rows[chunk_ptr[c] .. chunk_ptr[c+1]): ndot 1:
+// <v0|v1>; 2: <v0|v0>, <v0|v1>; 3: <v0|v0>, <v0|v1>, <v1|v1>, summed over the
+// chunk's rows and all R columns.
+hipError_t launch_group_dots(int64_t nchunks, const int64_t *chunk_ptr, const int32_t *rows,
+                             int R, int ndot, VecMode vm, const void *v0, const void *v1,
+                             double2 *partial, hipStream_t st);
+// out[d * G + g] = sum of group g's chunk partials (chunks grp_chunk[g] ..
+// grp_chunk[g+1]), fixed order.
+hipError_t launch_group_finish(int64_t G, const int64_t *grp_chunk, int ndot,
+                               const double2 *partial, double2 *out, hipStream_t st);
 
 // ---- graph-local S^{-1/2} patches (rg_isqrt.hip) ----
 hipError_t launch_patch_gather(int64_t first, int64_t count, int b, const int64_t *row_ptr,

@@ -35,12 +35,12 @@ constexpr int kMaxGrid = 2048;
 
 // XCD-contiguous row ranges in the MFMA SpMM (1) or plain round-robin (0).
 int g_xcd_map = 1;
-// Operand prefetch one K-step ahead in the MFMA SpMM.
-int g_mfma_prefetch = 0;
+// Output rows per epilogue pass of the MFMA SpMM (1, 2 or 4).
+int g_epi_unroll = 1;
 // Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
 int g_mfma_ntcap = 16;
 // Union-staged path for dense blocks (when the matrix carries group lists).
-int g_union_stage = 1;
+int g_union_stage = 0;
 // LDS bytes one union-staged workgroup stages per chunk (two groups per CU).
 constexpr size_t kGrpLdsBudget = 72 * 1024;
 
@@ -352,7 +352,7 @@ __device__ __forceinline__ void store_row(S *__restrict__ p, const S (&v)[NT]) {
     }
 }
 
-template <typename S, int NT, int MODE, bool CV, bool PF>
+template <typename S, int NT, int MODE, bool CV, int EPU>
 __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                                                    const int64_t *__restrict__ row_ptr,
                                                    const int32_t *__restrict__ col_ind,
@@ -419,35 +419,18 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                 for (int t = 0; t < NT; ++t) bv[t] = S(0);
             }
         };
-        if constexpr (PF) {
-            S a_cur, bv_cur[NT];
-            fetch(blk, kk, a_cur, bv_cur);
-            for (int st = 0; st < nsteps; ++st) {
-                int kn = kk + 4, bn = blk;
-                while (kn >= b) { kn -= b; ++bn; }
-                S a_nxt, bv_nxt[NT];
-                fetch(bn, kn, a_nxt, bv_nxt);
+        for (int st = 0; st < nsteps; ++st) {
+            S a, bv[NT];
+            fetch(blk, kk, a, bv);
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t] = mfma16(a_cur, bv_cur[t], acc[t]);
-                a_cur = a_nxt;
-#pragma unroll
-                for (int t = 0; t < NT; ++t) bv_cur[t] = bv_nxt[t];
-                kk = kn;
-                blk = bn;
-            }
-        } else {
-            for (int st = 0; st < nsteps; ++st) {
-                S a, bv[NT];
-                fetch(blk, kk, a, bv);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
-                kk += 4;
-                while (kk >= b) { kk -= b; ++blk; }
-            }
+            for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
+            kk += 4;
+            while (kk >= b) { kk -= b; ++blk; }
         }
 
-        // one output row per pass keeps the epilogue's registers low
-#pragma unroll 1
+        // EPU output rows per pass: 1 keeps the epilogue's registers low,
+        // more keeps more row loads in flight
+#pragma unroll EPU
         for (int v = 0; v < 4; ++v) {
             // accumulator element v holds row 4 v + g (f64) or 4 g + v (f32)
             const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
@@ -554,8 +537,8 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     }
 }
 
-template <typename S, int NT, bool CV, bool PF>
-hipError_t go_mfma_pf(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+template <typename S, int NT, bool CV, int EPU>
+hipError_t go_mfma_epu(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                       int grid, hipStream_t st) {
     const int xcd = g_xcd_map;
     const dim3 gd(grid), bd(64 * kWaves);
@@ -564,23 +547,23 @@ hipError_t go_mfma_pf(const BsrDev &A, const void *X, void *Y, int W, int R, con
     S *y = static_cast<S *>(Y);
     switch (e.mode) {
     case EPI_PLAIN:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV, EPU>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     case EPI_FIRST:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV, EPU>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     case EPI_STEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, PF>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, EPU>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     case EPI_PLAINSTEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, PF>), gd, bd, 0, st, A.nbr,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, EPU>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     default:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV, PF>), gd, bd, 0, st, A.nbr,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV, EPU>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     }
@@ -590,8 +573,11 @@ hipError_t go_mfma_pf(const BsrDev &A, const void *X, void *Y, int W, int R, con
 template <typename S, int NT, bool CV>
 hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                    int grid, hipStream_t st) {
-    return g_mfma_prefetch ? go_mfma_pf<S, NT, CV, true>(A, X, Y, W, R, e, grid, st)
-                           : go_mfma_pf<S, NT, CV, false>(A, X, Y, W, R, e, grid, st);
+    switch (g_epi_unroll) {
+    case 2: return go_mfma_epu<S, NT, CV, 2>(A, X, Y, W, R, e, grid, st);
+    case 4: return go_mfma_epu<S, NT, CV, 4>(A, X, Y, W, R, e, grid, st);
+    default: return go_mfma_epu<S, NT, CV, 1>(A, X, Y, W, R, e, grid, st);
+    }
 }
 
 template <typename S, bool CV>
@@ -880,10 +866,19 @@ __global__ __launch_bounds__(256) void k_reduce(const double2 *__restrict__ part
     const int u = pair - d * R;
     // compensated FP64 sums over the workgroup partials, then a fixed xor tree
     double sx = 0.0, cx = 0.0, sy = 0.0, cy = 0.0;
-    for (int gi = lane; gi < ngroups; gi += 64) {
-        const double2 p = partials[((int64_t)gi * ndot + d) * R + u];
-        two_sum_acc(sx, cx, p.x);
-        two_sum_acc(sy, cy, p.y);
+    // eight loads in flight per lane before they are summed in order
+    for (int g0 = lane; g0 < ngroups; g0 += 64 * 8) {
+        double2 p[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gi = g0 + 64 * q;
+            p[q] = gi < ngroups ? partials[((int64_t)gi * ndot + d) * R + u] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            two_sum_acc(sx, cx, p[q].x);
+            two_sum_acc(sy, cy, p[q].y);
+        }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -1059,9 +1054,9 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
 int spmm_max_grid() { return kMaxGrid; }
 
 void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
-void set_mfma_prefetch(int on) { g_mfma_prefetch = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_union_stage(int on) { g_union_stage = on ? 1 : 0; }
+void set_epi_unroll(int n) { g_epi_unroll = n; }
 int union_group_rows() { return kGrpWaves; }
 
 hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,

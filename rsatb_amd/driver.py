@@ -29,6 +29,7 @@ from . import _lib
 from .bsr import BSR
 from .lattice import Structure, edges_from_graph, neighbor_edges
 from .models import assemble
+from .pdos import pdos_groups_by_atom, pdos_groups_by_orbital_slot
 
 
 @dataclass
@@ -99,6 +100,7 @@ class RSATB:
         self._ctx = _lib.Context(device)
         self._kstates = []
         self._bounds = None
+        self.atom_index = np.arange(structure.n_atoms)
         self._setup()
 
     @classmethod
@@ -122,7 +124,9 @@ class RSATB:
         st = Structure(pos, np.asarray(cell, dtype=np.float64).reshape(3, 3), (True, True, True),
                        np.asarray(atom_type, dtype=np.int32))
         edges = edges_from_graph(st, edge_index, edge_shift_vec)
-        return cls(st, model, options, npar, device, edges=edges)
+        calc = cls(st, model, options, npar, device, edges=edges)
+        calc.atom_index = atom_index
+        return calc
 
     # ------------------------------------------------------------- setup
     def _setup(self) -> None:
@@ -232,6 +236,43 @@ class RSATB:
             dos, omega = self._ctx.compute_dos(emin, emax, dE, prec)
             parts.append(dos)
         return np.concatenate(parts), omega
+
+    def compute_pdos(self, dE: float, projection: str = "atom"):
+        """ComputePdos (rsatb/calc/observables.hpp:446-560) and its Python
+        return value (rsatb/pybind_rsatb.cpp:671-698): a dict with energy
+        (2nt,), pdos (n_k, n_group, 2nt), total (n_k, 2nt), n_group,
+        population, group_type and group_slot. Both operator forms here are
+        Hermitian, so the population is always 'lowdin'."""
+        if not (dE > 0.0):
+            raise RuntimeError("pdos dE must be a positive energy resolution in eV")
+        emin, emax = self.energy_bounds()
+        st, norb = self.structure, self.model.norb
+        type_norb = np.full(int(st.atom_type.max()) + 1 if st.n_atoms else 1, norb)
+        if projection == "atom":
+            groups = pdos_groups_by_atom(st.atom_type, type_norb, self.atom_index)
+        elif projection == "orbital_slot":
+            groups = pdos_groups_by_orbital_slot(st.atom_type, type_norb)
+        else:
+            raise RuntimeError(f"pdos projection must be 'atom' or 'orbital_slot', got '{projection}'")
+        nt, N = _lib.dos_plan(emin, emax, dE)
+        nt //= 2
+        x = np.concatenate([(np.arange(nt) - nt) / nt, np.arange(nt) / nt])
+        half, shift = (emax - emin) / 2.0, (emax + emin) / 2.0
+        prec = _lib.PREC_FP32 if self.options.precision == "fp32" else _lib.PREC_FP64
+        nk = len(self._kstates)
+        pdos = np.zeros((nk, groups.n_groups, 2 * nt))
+        total = np.zeros((nk, 2 * nt))
+        for k_idx in range(nk):
+            self._bind_kpoint(k_idx)
+            mu_t, mu_g = self._ctx.kpm_pdos_moments(N, groups.group_of_row, groups.n_groups, prec)
+            total[k_idx] = 2.0 * _lib.kpm_density(mu_t, x, 0.5 / nt)
+            for g in range(groups.n_groups):
+                pdos[k_idx, g] = 2.0 * _lib.kpm_density(mu_g[g], x, 0.5 / nt)
+        energy = np.concatenate([(np.arange(nt) - nt) * half / nt + shift, np.arange(nt) * half / nt + shift])
+        return {
+            "energy": energy, "pdos": pdos, "total": total, "n_group": groups.n_groups,
+            "population": "lowdin", "group_type": groups.group_type, "group_slot": groups.group_slot,
+        }
 
     def dump_operator_coo(self, which: str, axis: int = 0, k_idx: int = 0):
         """Global COO triplets of H or S (rsatb/pybind_rsatb.cpp:612-670)."""
