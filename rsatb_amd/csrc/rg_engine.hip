@@ -23,6 +23,11 @@ using namespace rg;
 
 namespace {
 
+// Bytes allocated past the end of every matrix value array: the DMA-staged
+// SpMM moves each K step's A fragment as 512 bytes from its first column, past
+// the end of the last block row (the excess is masked, the bytes must exist).
+constexpr size_t kValsPad = 1024;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -57,6 +62,9 @@ struct Slot {
     // union lists for the union-staged SpMM (dense blocks, sorted rows)
     DevBuf grp_ptr, grp_cols, pos;
     bool has_groups = false;
+    // gather table of the DMA-staged SpMM: xrow[s * b + kk] = col_ind[s] * b + kk
+    DevBuf xrow;
+    bool has_xrow = false;
 };
 
 struct KStat {
@@ -172,6 +180,7 @@ BsrDev dev_view(const Slot &s, bool fp32) {
     d.col_ind = static_cast<const int32_t *>(s.col_ind.p);
     d.vals = fp32 ? s.vals32.p : s.vals64.p;
     d.is_complex = s.is_complex;
+    if (s.has_xrow) d.xrow = static_cast<const int32_t *>(s.xrow.p);
     if (s.has_groups) {
         d.grp_ptr = static_cast<const int64_t *>(s.grp_ptr.p);
         d.grp_cols = static_cast<const int32_t *>(s.grp_cols.p);
@@ -188,6 +197,22 @@ double matrix_bytes(const Slot &s, bool fp32) {
 double matrix_flops(const Slot &s, VecMode vm, int R) {
     const double c = s.is_complex ? 4.0 : (vm.complex_vec ? 2.0 : 1.0);
     return 2.0 * (double)s.nnzb * s.b * s.b * (double)R * c;
+}
+
+// Gather table (see BsrDev::xrow) of a real slot with b >= 8, the blocks the
+// MFMA SpMMs take; 8 trailing entries keep the last row's padded step in range.
+int build_xrow(rg_context *ctx, Slot &s, const int32_t *ci) {
+    s.has_xrow = false;
+    if (s.b < 8 || s.is_complex) return RG_OK;
+    const int64_t n = s.nnzb * s.b;
+    if ((int64_t)s.nbr * s.b > INT32_MAX) return RG_OK;
+    std::vector<int32_t> t((size_t)n + 8, 0);
+    for (int64_t q = 0; q < s.nnzb; ++q)
+        for (int kk = 0; kk < s.b; ++kk) t[(size_t)(q * s.b + kk)] = ci[q] * s.b + kk;
+    RG_HIP(ctx, s.xrow.ensure(t.size() * 4), "hipMalloc gather table");
+    RG_HIP(ctx, hipMemcpy(s.xrow.p, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload gather table");
+    s.has_xrow = true;
+    return RG_OK;
 }
 
 // Union lists of consecutive groups of block rows (see BsrDev) for a slot
@@ -356,7 +381,7 @@ int ensure_fp32(rg_context *ctx) {
         Slot &s = ctx->slots[slots[i]];
         if (s.vals32_valid) continue;
         const int64_t n = s.nnzb * s.b * s.b * (s.is_complex ? 2 : 1);
-        RG_HIP(ctx, s.vals32.ensure((size_t)n * 4), "hipMalloc fp32 values");
+        RG_HIP(ctx, s.vals32.ensure((size_t)n * 4 + kValsPad), "hipMalloc fp32 values");
         RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.vals64.p),
                                       static_cast<float *>(s.vals32.p), n, ctx->stream),
                "fp32 conversion");
@@ -636,7 +661,7 @@ int rg_create(int device_id, rg_context **out) {
     if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
     if (const char *v = std::getenv("RG_UNION")) set_union_stage(std::atoi(v));
-    if (const char *v = std::getenv("RG_EPU")) set_epi_unroll(std::atoi(v));
+    if (const char *v = std::getenv("RG_DMA")) set_dma_depth(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RG_ERR_HIP;
@@ -662,6 +687,7 @@ void rg_destroy(rg_context *ctx) {
         s.grp_ptr.release();
         s.grp_cols.release();
         s.pos.release();
+        s.xrow.release();
     }
     ctx->psi.release();
     ctx->ckpt.release();
@@ -698,7 +724,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     const size_t nv = (size_t)nnzb * b * b * (dtype == RG_C128 ? 2 : 1);
     RG_HIP(ctx, s.row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc row_ptr");
     RG_HIP(ctx, s.col_ind.ensure((size_t)nnzb * 4 + 4), "hipMalloc col_ind");
-    RG_HIP(ctx, s.vals64.ensure(nv * 8 + 8), "hipMalloc values");
+    RG_HIP(ctx, s.vals64.ensure(nv * 8 + kValsPad), "hipMalloc values");
     RG_HIP(ctx, hipMemcpy(s.row_ptr.p, row_ptr, (size_t)(nbr + 1) * 8, hipMemcpyHostToDevice), "upload row_ptr");
     if (nnzb > 0) {
         RG_HIP(ctx, hipMemcpy(s.col_ind.p, col_ind, (size_t)nnzb * 4, hipMemcpyHostToDevice), "upload col_ind");
@@ -719,6 +745,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     s.nnzb = nnzb;
     s.is_complex = (dtype == RG_C128);
     int rcg = build_groups(ctx, s, row_ptr, col_ind);
+    if (!rcg) rcg = build_xrow(ctx, s, col_ind);
     if (rcg) return rcg;
     s.present = true;
     return RG_OK;
@@ -1016,7 +1043,7 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         t->vals32_valid = false;
         RG_HIP(ctx, t->row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc");
         RG_HIP(ctx, t->col_ind.ensure((size_t)S.nnzb * 4 + 4), "hipMalloc");
-        RG_HIP(ctx, t->vals64.ensure((size_t)S.nnzb * b * b * 8 + 8), "hipMalloc");
+        RG_HIP(ctx, t->vals64.ensure((size_t)S.nnzb * b * b * 8 + kValsPad), "hipMalloc");
         RG_HIP(ctx, hipMemcpy(t->row_ptr.p, S.row_ptr.p, (size_t)(nbr + 1) * 8, hipMemcpyDeviceToDevice), "copy");
         if (S.nnzb)
             RG_HIP(ctx, hipMemcpy(t->col_ind.p, S.col_ind.p, (size_t)S.nnzb * 4, hipMemcpyDeviceToDevice), "copy");
@@ -1025,6 +1052,7 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         t->nnzb = S.nnzb;
         t->is_complex = false;
         int rcg = build_groups(ctx, *t, rp.data(), ci.data());
+        if (!rcg) rcg = build_xrow(ctx, *t, ci.data());
         if (rcg) return rcg;
     }
     int it_max = 0;

@@ -22,6 +22,9 @@ struct BsrDev {
     const int64_t *grp_ptr = nullptr;
     const int32_t *grp_cols = nullptr;
     const int32_t *pos = nullptr;
+    // gather table of real b >= 8 matrices, or null: xrow[s * b + kk] =
+    // col_ind[s] * b + kk, the X row of column kk of block s (8 spare entries)
+    const int32_t *xrow = nullptr;
 };
 
 // Epilogue of a SpMM launch. y = sum_blocks A x is finished as
@@ -37,7 +40,7 @@ struct BsrDev {
 // laid out [gridDim.x][ndot][R].
 enum Epi { EPI_PLAIN = 0, EPI_FIRST = 1, EPI_STEP = 2, EPI_PLAINSTEP = 3, EPI_ADJ = 4 };
 
-inline int epi_ndot(int mode) {
+__host__ __device__ constexpr inline int epi_ndot(int mode) {
     return mode == EPI_FIRST ? 3 : mode == EPI_STEP ? 2 : mode == EPI_PLAINSTEP ? 1 : 0;
 }
 
@@ -67,7 +70,7 @@ int spmm_max_grid();
 void set_xcd_map(int on);
 void set_mfma_ntcap(int nt);
 void set_union_stage(int on);
-void set_epi_unroll(int n);
+void set_dma_depth(int p);
 int union_group_rows();
 
 // Device matrices hold every b x b block column-major; uploads arrive

@@ -35,12 +35,16 @@ constexpr int kMaxGrid = 2048;
 
 // XCD-contiguous row ranges in the MFMA SpMM (1) or plain round-robin (0).
 int g_xcd_map = 1;
-// Output rows per epilogue pass of the MFMA SpMM (1, 2 or 4).
-int g_epi_unroll = 1;
 // Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
 int g_mfma_ntcap = 16;
 // Union-staged path for dense blocks (when the matrix carries group lists).
 int g_union_stage = 0;
+// LDS-DMA staging depth P of the MFMA SpMM (k_spmm_dma); 0 keeps the
+// register-operand kernel k_spmm_mfma everywhere. With P > 0 the DMA kernel
+// runs every FP32 launch and the FP64 launches with a recurrence epilogue;
+// the FP64 plain product stays on k_spmm_mfma, which is faster there
+// (measured at the diamond 8^3 sandwich: 122 us against 140 us).
+int g_dma_depth = 2;
 // LDS bytes one union-staged workgroup stages per chunk (two groups per CU).
 constexpr size_t kGrpLdsBudget = 72 * 1024;
 
@@ -352,7 +356,137 @@ __device__ __forceinline__ void store_row(S *__restrict__ p, const S (&v)[NT]) {
     }
 }
 
-template <typename S, int NT, int MODE, bool CV, int EPU>
+// Epilogue of one finished 16-row tile of the MFMA SpMMs: the rows r < b of
+// block row br, columns col0 + i NT + t, from the accumulators; recurrence,
+// dots and store as EpiArgs says.
+template <typename S, int NT, int MODE, bool CV>
+__device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[NT], int64_t br,
+                                              int b, int W, int col0, int i, int g,
+                                              const EpiArgs &e,
+                                              double (&dacc)[epi_ndot(MODE) > 0 ? epi_ndot(MODE) : 1][NT],
+                                              S *__restrict__ Y) {
+    const S *__restrict__ XS = static_cast<const S *>(e.xs);
+    const S *__restrict__ XP = static_cast<const S *>(e.xp);
+    const S *__restrict__ PSI = static_cast<const S *>(e.psi);
+    // one output row per pass keeps the epilogue's registers low (unrolling
+    // it costs occupancy and time)
+#pragma unroll 1
+    for (int v = 0; v < 4; ++v) {
+        // accumulator element v holds row 4 v + g (f64) or 4 g + v (f32)
+        const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
+        if (r >= b) continue;
+        const int64_t base = (br * b + r) * (int64_t)W + col0 + i * NT;
+        S y[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            // element v of acc[t], selected without dynamic indexing
+            y[t] = v == 0 ? acc[t][0] : v == 1 ? acc[t][1] : v == 2 ? acc[t][2] : acc[t][3];
+        }
+        if constexpr (MODE != EPI_PLAIN) {
+            S xs[NT];
+            load_row<S, NT>(XS + base, xs);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
+            if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
+                S xp[NT];
+                load_row<S, NT>(XP + base, xp);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
+            }
+            if constexpr (MODE == EPI_ADJ) {
+                const S *qs[3] = {static_cast<const S *>(e.qm), static_cast<const S *>(e.qz),
+                                  static_cast<const S *>(e.qp)};
+                const double cs[3] = {e.cm, e.cz, e.cp};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    S qv[NT];
+                    load_row<S, NT>(qs[q] + base, qv);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
+                }
+            }
+            // dots (FP64): conj(a) b per complex column (t, t+1) or per
+            // real column; a self-dot is real and keeps no imaginary part
+            auto dot = [&](int d, const S (&a)[NT], const S (&bb)[NT], bool self) {
+                if constexpr (CV) {
+#pragma unroll
+                    for (int t = 0; t < NT; t += 2) {
+                        const double ar = a[t], ai = a[t + 1], br_ = bb[t], bi = bb[t + 1];
+                        dacc[d][t] = __builtin_fma(ar, br_, __builtin_fma(ai, bi, dacc[d][t]));
+                        if (!self)
+                            dacc[d][t + 1] = __builtin_fma(ar, bi, __builtin_fma(-ai, br_, dacc[d][t + 1]));
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        dacc[d][t] = __builtin_fma((double)a[t], (double)bb[t], dacc[d][t]);
+                }
+            };
+            if constexpr (MODE == EPI_FIRST) {
+                dot(0, xs, xs, true);
+                dot(1, xs, y, false);
+                dot(2, y, y, true);
+            } else if constexpr (MODE == EPI_STEP) {
+                dot(0, y, y, true);
+                dot(1, y, xs, false);
+            } else if constexpr (MODE == EPI_PLAINSTEP) {
+                S ps[NT];
+                load_row<S, NT>(PSI + base, ps);
+                dot(0, ps, y, false);
+            }
+        }
+        store_row<S, NT>(Y + base, y);
+    }
+}
+
+// Per-workgroup dot partials of the MFMA SpMMs: the lane groups of every
+// column are folded, then the waves sharing a column chunk, into
+// e.partials[(blockIdx.x * NDOT + d) * R + u].
+template <typename S, int NT, int MODE, bool CV>
+__device__ __forceinline__ void mfma_dot_partials(
+    double (&dacc)[epi_ndot(MODE) > 0 ? epi_ndot(MODE) : 1][NT], int wave, int i, int g,
+    int nchunks, int R, const EpiArgs &e) {
+    constexpr int NDOT = epi_ndot(MODE);
+    constexpr int CW = 16 * NT;
+    if constexpr (NDOT > 0) {
+        // fold the four lane groups (rows of the 16-row tile) of every column
+#pragma unroll
+        for (int d = 0; d < NDOT; ++d)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                dacc[d][t] += __shfl_xor(dacc[d][t], 16);
+                dacc[d][t] += __shfl_xor(dacc[d][t], 32);
+            }
+        // per wave: CW real columns -> CW/2 complex (or CW real) units
+        constexpr int UPC = CV ? CW / 2 : CW;
+        __shared__ double2 sm[kWaves][UPC];
+        for (int d = 0; d < NDOT; ++d) {
+            if (g == 0) {
+#pragma unroll
+                for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
+                    const int uu = CV ? (i * NT + t) / 2 : i * NT + t;
+                    sm[wave][uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
+                }
+            }
+            __syncthreads();
+            for (int u = threadIdx.x; u < R; u += blockDim.x) {
+                const int ch = (u * (CV ? 2 : 1)) / CW;
+                const int uu = u - ch * UPC;
+                double2 acc2 = double2{0.0, 0.0};
+                for (int w = ch; w < kWaves; w += nchunks) {
+                    acc2.x += sm[w][uu].x;
+                    acc2.y += sm[w][uu].y;
+                }
+                e.partials[((int64_t)blockIdx.x * NDOT + d) * R + u] = acc2;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename S, int NT, int MODE, bool CV>
 __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                                                    const int64_t *__restrict__ row_ptr,
                                                    const int32_t *__restrict__ col_ind,
@@ -362,9 +496,6 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     using A4 = typename Acc4<S>::T;
     constexpr int NDOT = MODE == EPI_FIRST ? 3 : MODE == EPI_STEP ? 2 : MODE == EPI_PLAINSTEP ? 1 : 0;
     constexpr int CW = 16 * NT; // real columns per chunk
-    const S *__restrict__ XS = static_cast<const S *>(e.xs);
-    const S *__restrict__ XP = static_cast<const S *>(e.xp);
-    const S *__restrict__ PSI = static_cast<const S *>(e.psi);
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -428,117 +559,293 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
             while (kk >= b) { kk -= b; ++blk; }
         }
 
-        // EPU output rows per pass: 1 keeps the epilogue's registers low,
-        // more keeps more row loads in flight
-#pragma unroll EPU
-        for (int v = 0; v < 4; ++v) {
-            // accumulator element v holds row 4 v + g (f64) or 4 g + v (f32)
-            const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
-            if (r >= b) continue;
-            const int64_t base = (br * b + r) * (int64_t)W + col0 + i * NT;
-            S y[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                // element v of acc[t], selected without dynamic indexing
-                y[t] = v == 0 ? acc[t][0] : v == 1 ? acc[t][1] : v == 2 ? acc[t][2] : acc[t][3];
-            }
-            if constexpr (MODE != EPI_PLAIN) {
-                S xs[NT];
-                load_row<S, NT>(XS + base, xs);
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
-                if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
-                    S xp[NT];
-                    load_row<S, NT>(XP + base, xp);
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
-                }
-                if constexpr (MODE == EPI_ADJ) {
-                    const S *qs[3] = {static_cast<const S *>(e.qm), static_cast<const S *>(e.qz),
-                                      static_cast<const S *>(e.qp)};
-                    const double cs[3] = {e.cm, e.cz, e.cp};
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        S qv[NT];
-                        load_row<S, NT>(qs[q] + base, qv);
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
-                    }
-                }
-                // dots (FP64): conj(a) b per complex column (t, t+1) or per
-                // real column; a self-dot is real and keeps no imaginary part
-                auto dot = [&](int d, const S (&a)[NT], const S (&bb)[NT], bool self) {
-                    if constexpr (CV) {
-#pragma unroll
-                        for (int t = 0; t < NT; t += 2) {
-                            const double ar = a[t], ai = a[t + 1], br_ = bb[t], bi = bb[t + 1];
-                            dacc[d][t] = __builtin_fma(ar, br_, __builtin_fma(ai, bi, dacc[d][t]));
-                            if (!self)
-                                dacc[d][t + 1] = __builtin_fma(ar, bi, __builtin_fma(-ai, br_, dacc[d][t + 1]));
-                        }
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            dacc[d][t] = __builtin_fma((double)a[t], (double)bb[t], dacc[d][t]);
-                    }
-                };
-                if constexpr (MODE == EPI_FIRST) {
-                    dot(0, xs, xs, true);
-                    dot(1, xs, y, false);
-                    dot(2, y, y, true);
-                } else if constexpr (MODE == EPI_STEP) {
-                    dot(0, y, y, true);
-                    dot(1, y, xs, false);
-                } else if constexpr (MODE == EPI_PLAINSTEP) {
-                    S ps[NT];
-                    load_row<S, NT>(PSI + base, ps);
-                    dot(0, ps, y, false);
-                }
-            }
-            store_row<S, NT>(Y + base, y);
-        }
+        mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
     }
 
-    if constexpr (NDOT > 0) {
-        // fold the four lane groups (rows of the 16-row tile) of every column
-#pragma unroll
-        for (int d = 0; d < NDOT; ++d)
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                dacc[d][t] += __shfl_xor(dacc[d][t], 16);
-                dacc[d][t] += __shfl_xor(dacc[d][t], 32);
-            }
-        // per wave: CW real columns -> CW/2 complex (or CW real) units
-        constexpr int UPC = CV ? CW / 2 : CW;
-        __shared__ double2 sm[kWaves][UPC];
-        for (int d = 0; d < NDOT; ++d) {
-            if (g == 0) {
-#pragma unroll
-                for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
-                    const int uu = CV ? (i * NT + t) / 2 : i * NT + t;
-                    sm[wave][uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
-                }
-            }
-            __syncthreads();
-            for (int u = threadIdx.x; u < R; u += blockDim.x) {
-                const int ch = (u * (CV ? 2 : 1)) / CW;
-                const int uu = u - ch * UPC;
-                double2 acc2 = double2{0.0, 0.0};
-                for (int w = ch; w < kWaves; w += nchunks) {
-                    acc2.x += sm[w][uu].x;
-                    acc2.y += sm[w][uu].y;
-                }
-                e.partials[((int64_t)blockIdx.x * NDOT + d) * R + u] = acc2;
-            }
-            __syncthreads();
-        }
-    }
+    mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
 }
 
-template <typename S, int NT, bool CV, int EPU>
-hipError_t go_mfma_epu(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+// ------------------------------------------------ DMA-staged MFMA SpMM
+// The arithmetic of k_spmm_mfma, with the operands of every 4-wide K step
+// (the four gathered X row chunks and the wave's 16 x 4 A fragment) moved
+// global -> LDS by the LDS-DMA path (global_load_lds, no VGPRs held while in
+// flight), P - 1 steps ahead of the MFMAs that consume them. The stream of
+// steps runs across the wave's block rows, so the next row's first steps are
+// in flight during a row's epilogue. Each wave owns P step buffers of its own
+// LDS; only vmcnt orders a buffer's DMA before its reads, and a buffer is
+// refilled one step after it was read (the read waits on lgkmcnt), so no
+// barrier is needed.
+
+// s_waitcnt vmcnt(VM), then N16 16-byte B chunks (at xaddr + 256 q) and the A
+// element (at aaddr, AB bytes) from LDS; the LDS reads complete (lgkmcnt(0))
+// before the asm ends, so its outputs are ready when it returns.
+using F4 = float __attribute__((ext_vector_type(4)));
+template <int VM, int N16, int AB> struct DmaRead;
+template <int VM> struct DmaRead<VM, 1, 8> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[1], double &a) {
+        asm volatile("s_waitcnt vmcnt(%3)\n\t"
+                     "ds_read_b128 %0, %2\n\t"
+                     "ds_read_b64 %1, %4\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 2, 8> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[2], double &a) {
+        asm volatile("s_waitcnt vmcnt(%4)\n\t"
+                     "ds_read_b128 %0, %3\n\t"
+                     "ds_read_b128 %1, %3 offset:256\n\t"
+                     "ds_read_b64 %2, %5\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 4, 8> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[4], double &a) {
+        asm volatile("s_waitcnt vmcnt(%6)\n\t"
+                     "ds_read_b128 %0, %5\n\t"
+                     "ds_read_b128 %1, %5 offset:256\n\t"
+                     "ds_read_b128 %2, %5 offset:512\n\t"
+                     "ds_read_b128 %3, %5 offset:768\n\t"
+                     "ds_read_b64 %4, %7\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 8, 8> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[8], double &a) {
+        asm volatile("s_waitcnt vmcnt(%10)\n\t"
+                     "ds_read_b128 %0, %9\n\t"
+                     "ds_read_b128 %1, %9 offset:256\n\t"
+                     "ds_read_b128 %2, %9 offset:512\n\t"
+                     "ds_read_b128 %3, %9 offset:768\n\t"
+                     "ds_read_b128 %4, %9 offset:1024\n\t"
+                     "ds_read_b128 %5, %9 offset:1280\n\t"
+                     "ds_read_b128 %6, %9 offset:1536\n\t"
+                     "ds_read_b128 %7, %9 offset:1792\n\t"
+                     "ds_read_b64 %8, %11\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 1, 4> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[1], float &a) {
+        asm volatile("s_waitcnt vmcnt(%3)\n\t"
+                     "ds_read_b128 %0, %2\n\t"
+                     "ds_read_b32 %1, %4\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 2, 4> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[2], float &a) {
+        asm volatile("s_waitcnt vmcnt(%4)\n\t"
+                     "ds_read_b128 %0, %3\n\t"
+                     "ds_read_b128 %1, %3 offset:256\n\t"
+                     "ds_read_b32 %2, %5\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 4, 4> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[4], float &a) {
+        asm volatile("s_waitcnt vmcnt(%6)\n\t"
+                     "ds_read_b128 %0, %5\n\t"
+                     "ds_read_b128 %1, %5 offset:256\n\t"
+                     "ds_read_b128 %2, %5 offset:512\n\t"
+                     "ds_read_b128 %3, %5 offset:768\n\t"
+                     "ds_read_b32 %4, %7\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+template <int VM> struct DmaRead<VM, 8, 4> {
+    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[8], float &a) {
+        asm volatile("s_waitcnt vmcnt(%10)\n\t"
+                     "ds_read_b128 %0, %9\n\t"
+                     "ds_read_b128 %1, %9 offset:256\n\t"
+                     "ds_read_b128 %2, %9 offset:512\n\t"
+                     "ds_read_b128 %3, %9 offset:768\n\t"
+                     "ds_read_b128 %4, %9 offset:1024\n\t"
+                     "ds_read_b128 %5, %9 offset:1280\n\t"
+                     "ds_read_b128 %6, %9 offset:1536\n\t"
+                     "ds_read_b128 %7, %9 offset:1792\n\t"
+                     "ds_read_b32 %8, %11\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(a)
+                     : "v"(xaddr), "i"(VM), "v"(aaddr)
+                     : "memory");
+    }
+};
+
+// 32-bit LDS byte address of a pointer into __shared__ memory
+__device__ __forceinline__ unsigned lds_addr(const void *p) {
+    return (unsigned)(uintptr_t)((const __attribute__((address_space(3))) char *)p);
+}
+#define RG_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+
+template <typename S, int NT>
+__host__ __device__ constexpr int dma_step_bytes() {
+    return 4 * 16 * NT * (int)sizeof(S) + 64 * (int)sizeof(S);
+}
+
+template <typename S, int NT, int MODE, bool CV, int P>
+__global__ __launch_bounds__(256) void k_spmm_dma(int64_t nbr, int b,
+                                                  const int64_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ xrow,
+                                                  const S *__restrict__ vals,
+                                                  const S *__restrict__ X, S *__restrict__ Y,
+                                                  int W, int R, int xcd_map, EpiArgs e) {
+    using A4 = typename Acc4<S>::T;
+    constexpr int NDOT = epi_ndot(MODE);
+    constexpr int CW = 16 * NT;
+    constexpr int ROWB = CW * (int)sizeof(S); // bytes of one gathered X row chunk
+    constexpr int XB = 4 * ROWB;              // X bytes per step
+    static_assert(XB % 1024 == 0, "one X DMA instruction moves 1 KB");
+    constexpr int NI = XB / 1024;             // X DMA instructions per step
+    constexpr int NA = (int)sizeof(S) / 4;    // A DMA instructions per step (256 B each)
+    constexpr int STB = dma_step_bytes<S, NT>();
+    constexpr int VM = NI + NA;               // vmcnt increments per step
+    static_assert(VM * (P - 1) <= 63, "vmcnt holds 6 bits");
+    constexpr int N16 = NT * (int)sizeof(S) / 16;
+    static_assert(N16 >= 1, "B chunk of at least 16 bytes");
+    extern __shared__ __attribute__((aligned(16))) unsigned char dma_lds[];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = lane & 15;
+    const int g = lane >> 4;
+    const int nchunks = W / CW; // 1, 2 or 4 (divides kWaves): a wave keeps chunk wave % nchunks
+    const int cshift = nchunks == 4 ? 2 : nchunks == 2 ? 1 : 0;
+    const int col0 = (wave % nchunks) * CW;
+    int64_t r_lo = 0, r_hi = nbr;
+    int64_t kq = blockIdx.x, nk = gridDim.x;
+    if (xcd_map) { // as k_spmm_mfma: each XCD walks its own eighth of the rows
+        const int x = blockIdx.x & 7;
+        kq = blockIdx.x >> 3;
+        nk = gridDim.x >> 3;
+        r_lo = nbr * x / 8;
+        r_hi = nbr * (x + 1) / 8;
+    }
+    const int64_t total = (r_hi - r_lo) * nchunks;
+    const int64_t wstride = nk * kWaves;
+    unsigned char *wbuf = dma_lds + wave * (P * STB);
+    const unsigned wbuf_addr = lds_addr(wbuf);
+    const char *Xc = reinterpret_cast<const char *>(X) + (size_t)col0 * sizeof(S);
+    const char *Vc = reinterpret_cast<const char *>(vals);
+    // LDS order of a row chunk: 16-byte chunk c = i N16 + q of the row (lane
+    // i's q-th chunk) sits at position q 16 + i, so each of a lane group's N16
+    // ds_read_b128 touches 256 contiguous bytes (no bank conflicts). The DMA
+    // lane filling position pos of the step's X region therefore fetches row
+    // pos / (16 N16), chunk ((pp & 15) N16 + pp / 16) of it (pp: position in
+    // the row).
+
+    // ---- issue side: the wave's K steps in consumption order. Within a
+    // block row the blocks are consecutive and column-major, so the row's A
+    // panel is one column-major b x K matrix: step st's four columns are the
+    // 4 b contiguous elements from (s0 b + 4 st) b. The step's X rows come
+    // from the gather table at s0 b + 4 st.
+    int64_t iit = kq * kWaves + wave; // work item whose steps are being issued
+    int64_t is0 = 0;                  // its first block
+    int insteps = 0, ist = 0;
+    auto open_row = [&]() { // from iit, the next work item with at least one step
+        while (iit < total) {
+            const int64_t br = r_lo + (iit >> cshift);
+            is0 = row_ptr[br];
+            insteps = ((int)(row_ptr[br + 1] - is0) * b + 3) >> 2;
+            if (insteps > 0) break;
+            iit += wstride;
+        }
+        ist = 0;
+    };
+    open_row();
+    // DMAs the next step into buffer `slot`; false once the stream is done
+    auto issue = [&](int slot) -> bool {
+        if (iit >= total) return false;
+        unsigned char *buf = wbuf + slot * STB;
+        const int64_t kb = is0 * b + 4 * ist;
+        int64_t xr[4];
+#pragma unroll
+        for (int kr = 0; kr < 4; ++kr) xr[kr] = (int64_t)xrow[kb + kr] * W;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            // instruction j fills positions [64 j, 64 j + 64) of the step
+            const int pos = j * 64 + lane;
+            const int kr = pos / (16 * N16);
+            const int pp = pos - kr * (16 * N16);
+            const int src = ((pp & 15) * N16 + (pp >> 4)) * 16;
+            int64_t r;
+            if constexpr (ROWB >= 1024) r = xr[(j * 1024) / ROWB];
+            else r = kr == 0 ? xr[0] : kr == 1 ? xr[1] : kr == 2 ? xr[2] : xr[3];
+            __builtin_amdgcn_global_load_lds(Xc + r * (int64_t)sizeof(S) + src,
+                                             RG_LDS(buf + j * 1024), 16, 0, 0);
+        }
+        const char *asrc = Vc + (kb * b) * (int64_t)sizeof(S) + 4 * lane;
+#pragma unroll
+        for (int q = 0; q < NA; ++q)
+            __builtin_amdgcn_global_load_lds(asrc + 256 * q, RG_LDS(buf + XB + 256 * q), 4, 0, 0);
+        if (++ist == insteps) {
+            iit += wstride;
+            open_row();
+        }
+        return true;
+    };
+
+    double dacc[NDOT > 0 ? NDOT : 1][NT];
+#pragma unroll
+    for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
+
+#pragma unroll
+    for (int q = 0; q < P - 1; ++q) (void)issue(q);
+    int cs = 0; // step buffer consumed next
+    const unsigned xoff = g * ROWB + i * 16;
+    for (int64_t it = kq * kWaves + wave; it < total; it += wstride) {
+        const int64_t br = r_lo + (it >> cshift);
+        const int K = (int)(row_ptr[br + 1] - row_ptr[br]) * b;
+        const int nsteps = (K + 3) >> 2;
+        A4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = A4{0, 0, 0, 0};
+        for (int st = 0; st < nsteps; ++st) {
+            // refill the buffer the previous step read, then wait for this one
+            const bool more = issue(cs == 0 ? P - 1 : cs - 1);
+            const unsigned sb = wbuf_addr + cs * STB;
+            const unsigned xa = sb + xoff;
+            const unsigned aa = sb + XB + (g * b + i) * (int)sizeof(S);
+            F4 o[N16];
+            S a;
+            if (more) DmaRead<VM * (P - 1), N16, (int)sizeof(S)>::run(xa, aa, o, a);
+            else DmaRead<0, N16, (int)sizeof(S)>::run(xa, aa, o, a);
+            if (!(i < b && 4 * st + g < K)) a = S(0);
+            S bv[NT];
+            __builtin_memcpy(bv, o, sizeof(bv));
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
+            cs = cs + 1 == P ? 0 : cs + 1;
+        }
+        mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
+    }
+    mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
+}
+
+template <typename S, int NT, bool CV>
+hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                       int grid, hipStream_t st) {
     const int xcd = g_xcd_map;
     const dim3 gd(grid), bd(64 * kWaves);
@@ -547,36 +854,77 @@ hipError_t go_mfma_epu(const BsrDev &A, const void *X, void *Y, int W, int R, co
     S *y = static_cast<S *>(Y);
     switch (e.mode) {
     case EPI_PLAIN:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV, EPU>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     case EPI_FIRST:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV, EPU>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     case EPI_STEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, EPU>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     case EPI_PLAINSTEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, EPU>), gd, bd, 0, st, A.nbr,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     default:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV, EPU>), gd, bd, 0, st, A.nbr,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
         break;
     }
     return hipGetLastError();
 }
 
+template <typename S, int NT, int MODE, bool CV, int P>
+hipError_t go_dma_mode(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                       int grid, hipStream_t st) {
+    constexpr unsigned lds = kWaves * P * dma_step_bytes<S, NT>();
+    static bool attr_set = false; // one attribute call per kernel instance
+    if (!attr_set) {
+        hipError_t he = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&k_spmm_dma<S, NT, MODE, CV, P>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (he != hipSuccess) return he;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_spmm_dma<S, NT, MODE, CV, P>), dim3(grid), dim3(64 * kWaves), lds, st,
+                       A.nbr, (int)A.b, A.row_ptr, A.xrow, static_cast<const S *>(A.vals),
+                       static_cast<const S *>(X), static_cast<S *>(Y), W, R, g_xcd_map, e);
+    return hipGetLastError();
+}
+
+template <typename S, int NT, bool CV, int P>
+hipError_t go_dma_p(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                    int grid, hipStream_t st) {
+    switch (e.mode) {
+    case EPI_PLAIN: return go_dma_mode<S, NT, EPI_PLAIN, CV, P>(A, X, Y, W, R, e, grid, st);
+    case EPI_FIRST: return go_dma_mode<S, NT, EPI_FIRST, CV, P>(A, X, Y, W, R, e, grid, st);
+    case EPI_STEP: return go_dma_mode<S, NT, EPI_STEP, CV, P>(A, X, Y, W, R, e, grid, st);
+    case EPI_PLAINSTEP: return go_dma_mode<S, NT, EPI_PLAINSTEP, CV, P>(A, X, Y, W, R, e, grid, st);
+    default: return go_dma_mode<S, NT, EPI_ADJ, CV, P>(A, X, Y, W, R, e, grid, st);
+    }
+}
+
 template <typename S, int NT, bool CV>
-hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
-                   int grid, hipStream_t st) {
-    switch (g_epi_unroll) {
-    case 2: return go_mfma_epu<S, NT, CV, 2>(A, X, Y, W, R, e, grid, st);
-    case 4: return go_mfma_epu<S, NT, CV, 4>(A, X, Y, W, R, e, grid, st);
-    default: return go_mfma_epu<S, NT, CV, 1>(A, X, Y, W, R, e, grid, st);
+hipError_t go_dma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e, int grid,
+                  hipStream_t st) {
+    if (g_dma_depth == 4) return go_dma_p<S, NT, CV, 4>(A, X, Y, W, R, e, grid, st);
+    return go_dma_p<S, NT, CV, 2>(A, X, Y, W, R, e, grid, st);
+}
+
+// NT * sizeof(S) >= 16: FP64 with NT >= 2, FP32 with NT >= 4
+template <typename S, bool CV>
+hipError_t dma_by_nt(int nt, const BsrDev &A, const void *X, void *Y, int W, int R,
+                     const EpiArgs &e, int grid, hipStream_t st) {
+    switch (nt) {
+    case 2:
+        if constexpr (sizeof(S) == 8) return go_dma<S, 2, CV>(A, X, Y, W, R, e, grid, st);
+        return hipErrorInvalidValue;
+    case 4: return go_dma<S, 4, CV>(A, X, Y, W, R, e, grid, st);
+    case 8: return go_dma<S, 8, CV>(A, X, Y, W, R, e, grid, st);
+    default: return go_dma<S, 16, CV>(A, X, Y, W, R, e, grid, st);
     }
 }
 
@@ -1032,6 +1380,14 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
         g = ((g + 7) / 8) * 8; // whole XCD rounds (kMaxGrid is a multiple of 8)
         const int grid = (int)g;
         if (grid_out) *grid_out = grid;
+        if (g_dma_depth > 0 && A.xrow && nt * (vm.fp32 ? 4 : 8) >= 16 &&
+            (vm.fp32 || e.mode != EPI_PLAIN)) {
+            if (vm.fp32)
+                return vm.complex_vec ? dma_by_nt<float, true>(nt, A, X, Y, W, R, e, grid, stream)
+                                      : dma_by_nt<float, false>(nt, A, X, Y, W, R, e, grid, stream);
+            return vm.complex_vec ? dma_by_nt<double, true>(nt, A, X, Y, W, R, e, grid, stream)
+                                  : dma_by_nt<double, false>(nt, A, X, Y, W, R, e, grid, stream);
+        }
         if (vm.fp32) {
             return vm.complex_vec ? by_nt<float, true>(A, X, Y, W, R, e, grid, stream)
                                   : by_nt<float, false>(A, X, Y, W, R, e, grid, stream);
@@ -1056,7 +1412,7 @@ int spmm_max_grid() { return kMaxGrid; }
 void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_union_stage(int on) { g_union_stage = on ? 1 : 0; }
-void set_epi_unroll(int n) { g_epi_unroll = n; }
+void set_dma_depth(int p) { g_dma_depth = (p == 0 || p == 4) ? p : 2; }
 int union_group_rows() { return kGrpWaves; }
 
 hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
