@@ -50,7 +50,29 @@ def parse():
     return p.parse_args()
 
 
-def roofline(stats: dict, precision: str) -> dict:
+PROFILES = Path(__file__).resolve().parent / "profiles"
+
+
+def measured_traffic(kclass: str, precision: str, workload_tag: str):
+    """HBM bytes per launch of a kernel class, from the committed rocprofv3
+    counter summary of this workload and precision (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy kernel), or None when no summary covers it."""
+    try:
+        rec = json.loads((PROFILES / f"kpm_{workload_tag}_{precision}.json").read_text())
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload_tag") != workload_tag:
+        return None
+    dtype = "f32" if precision == "fp32" else "f64"
+    best = None
+    for k in rec.get("kernels", {}).values():
+        if k.get("class") == kclass and k.get("dtype") == dtype and "traffic_bytes" in k:
+            if best is None or k["calls"] > best["calls"]:
+                best = k
+    return None if best is None else round(best["traffic_bytes"])
+
+
+def roofline(stats: dict, precision: str, workload_tag: str = "") -> dict:
     """The dominant kernel class (largest total time) against its roofline."""
     best = max((s for s in stats.values() if s["count"] > 0), key=lambda s: s["avg_ms"] * s["count"])
     t = best["avg_ms"] * 1e-3
@@ -64,7 +86,8 @@ def roofline(stats: dict, precision: str) -> dict:
     else:
         out = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    out.update({"traffic": None, "kernel": best["name"], "avg_ms": round(best["avg_ms"], 5),
+    out.update({"traffic": measured_traffic(best["name"], precision, workload_tag), "kernel": best["name"],
+                "avg_ms": round(best["avg_ms"], 5),
                 "algorithmic_bytes": best["bytes"], "algorithmic_flops": best["flops"],
                 "intensity_flop_per_byte": round(intensity, 3), "achieved_GBs": round(gbs, 1),
                 "achieved_TFLOPs": round(tfs, 3)})
@@ -212,7 +235,7 @@ def main() -> int:
                                f"random-phase complex vectors, M={M} moments",
                    "n_orbitals": rows, "R": args.nsamples, "M": M, "form": "sandwich",
                    "precision": args.precision, "bounds": list(calc.energy_bounds())},
-        "roofline": roofline(stats, args.precision),
+        "roofline": roofline(stats, args.precision, f"diamond{args.cells}_R{args.nsamples}_sandwich"),
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "count": v["count"]} for k, v in stats.items()},
         "mu0": float(mu[0].real), "max_abs_mu": float(np.max(np.abs(mu))),
     }
