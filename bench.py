@@ -42,7 +42,8 @@ def parse():
     p.add_argument("--cells", type=int, default=8, help="diamond conventional cells per edge")
     p.add_argument("--nsamples", type=int, default=64)
     p.add_argument("--moments", type=int, default=2048)
-    p.add_argument("--cpu-moments", type=int, default=32, help="moments in the timed CPU sample")
+    p.add_argument("--cpu-moments", type=int, default=256,
+                   help="moments in the timed CPU sample (about 15 s on 16 host cores)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward leg")
