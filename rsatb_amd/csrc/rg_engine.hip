@@ -23,10 +23,12 @@ using namespace rg;
 
 namespace {
 
-// Bytes allocated past the end of every matrix value array: the DMA-staged
-// SpMM moves each K step's A fragment as 512 bytes from its first column, past
-// the end of the last block row (the excess is masked, the bytes must exist).
-constexpr size_t kValsPad = 1024;
+// Bytes allocated past the end of every matrix value array: the MFMA SpMMs
+// read A fragments up to 8 K columns past a row's panel (the DMA kernel moves
+// 512 bytes from a step's first column; the register kernel prefetches one
+// step ahead); past the last block row those reads leave the values proper,
+// their lanes are masked, and the bytes must exist.
+constexpr size_t kValsPad = 2048;
 
 struct DevBuf {
     void *p = nullptr;
@@ -200,13 +202,14 @@ double matrix_flops(const Slot &s, VecMode vm, int R) {
 }
 
 // Gather table (see BsrDev::xrow) of a real slot with b >= 8, the blocks the
-// MFMA SpMMs take; 8 trailing entries keep the last row's padded step in range.
+// MFMA SpMMs take; 16 trailing entries keep the steps the kernels fetch past
+// the last row's K (at most 11 entries) in range.
 int build_xrow(rg_context *ctx, Slot &s, const int32_t *ci) {
     s.has_xrow = false;
     if (s.b < 8 || s.is_complex) return RG_OK;
     const int64_t n = s.nnzb * s.b;
     if ((int64_t)s.nbr * s.b > INT32_MAX) return RG_OK;
-    std::vector<int32_t> t((size_t)n + 8, 0);
+    std::vector<int32_t> t((size_t)n + 16, 0);
     for (int64_t q = 0; q < s.nnzb; ++q)
         for (int kk = 0; kk < s.b; ++kk) t[(size_t)(q * s.b + kk)] = ci[q] * s.b + kk;
     RG_HIP(ctx, s.xrow.ensure(t.size() * 4), "hipMalloc gather table");

@@ -23,7 +23,7 @@ struct BsrDev {
     const int32_t *grp_cols = nullptr;
     const int32_t *pos = nullptr;
     // gather table of real b >= 8 matrices, or null: xrow[s * b + kk] =
-    // col_ind[s] * b + kk, the X row of column kk of block s (8 spare entries)
+    // col_ind[s] * b + kk, the X row of column kk of block s (16 spare entries)
     const int32_t *xrow = nullptr;
 };
 

@@ -41,9 +41,9 @@ int g_mfma_ntcap = 16;
 int g_union_stage = 0;
 // LDS-DMA staging depth P of the MFMA SpMM (k_spmm_dma); 0 keeps the
 // register-operand kernel k_spmm_mfma everywhere. With P > 0 the DMA kernel
-// runs every FP32 launch and the FP64 launches with a recurrence epilogue;
-// the FP64 plain product stays on k_spmm_mfma, which is faster there
-// (measured at the diamond 8^3 sandwich: 122 us against 140 us).
+// runs the FP32 launches with a recurrence epilogue; the rest stay on
+// k_spmm_mfma, faster there (diamond 8^3 sandwich, R = 64 complex: FP64 plain
+// 113 us vs 140, FP64 step 142 vs 163, FP32 plain 62 vs 65; FP32 step 85 vs 92).
 int g_dma_depth = 2;
 // LDS bytes one union-staged workgroup stages per chunk (two groups per CU).
 constexpr size_t kGrpLdsBudget = 72 * 1024;
@@ -489,7 +489,7 @@ __device__ __forceinline__ void mfma_dot_partials(
 template <typename S, int NT, int MODE, bool CV>
 __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                                                    const int64_t *__restrict__ row_ptr,
-                                                   const int32_t *__restrict__ col_ind,
+                                                   const int32_t *__restrict__ xrow,
                                                    const S *__restrict__ vals,
                                                    const S *__restrict__ X, S *__restrict__ Y,
                                                    int W, int R, int xcd_map, EpiArgs e) {
@@ -534,29 +534,54 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t] = A4{0, 0, 0, 0};
 
-        // this lane's position k = 4 st + g in the concatenated K range; the
-        // operands of step st + 1 are fetched while step st's MFMAs run
-        int blk = 0, kk = g;
-        while (kk >= b) { kk -= b; ++blk; }
-        auto fetch = [&](int bl, int k, S &a, S (&bv)[NT]) {
-            a = S(0);
-            if (bl < nblk) {
-                const int64_t sb = s0 + bl;
-                if (i < b) a = __builtin_nontemporal_load(vals + sb * (int64_t)(b * b) + k * b + i);
-                const int64_t xr = (int64_t)col_ind[sb] * b + k;
-                load_row<S, NT>(X + xr * W + col0 + i * NT, bv);
-            } else {
-#pragma unroll
-                for (int t = 0; t < NT; ++t) bv[t] = S(0);
-            }
+        // Step st covers K columns 4 st .. 4 st + 3 of the row's column-major
+        // b x K panel (element (i, k) at s0 b^2 + k b + i); lane group g takes
+        // column k = 4 st + g, its X row from the gather table (wave-uniform
+        // scalar loads, so no vector load sits between the steps' loads). Two
+        // register buffers: step st + 1's operands are in flight while step
+        // st's MFMAs run. Past the row's K the table still names a valid row
+        // and the panel read stays inside the padded value array; a = 0 there.
+        // Lanes outside the panel (i >= b, or k >= K) load their A element from
+        // row_ptr[0], which is 0 (checked at upload): the 8 zero bytes read as
+        // +0.0, so no select on loaded data holds up the next loads.
+        const int K = nblk * b;
+        const S *__restrict__ panel = vals + s0 * (int64_t)b * b;
+        const S *__restrict__ zero = reinterpret_cast<const S *>(row_ptr);
+        int fst = 0; // next step to fetch
+        int64_t kb = s0 * b;
+        int32_t nx0 = xrow[kb], nx1 = xrow[kb + 1], nx2 = xrow[kb + 2], nx3 = xrow[kb + 3];
+        auto fetch = [&](S &a, S (&bv)[NT]) {
+            const int64_t xr = g == 0 ? nx0 : g == 1 ? nx1 : g == 2 ? nx2 : nx3;
+            const int k = 4 * fst + g;
+            a = __builtin_nontemporal_load((i < b && k < K) ? panel + (int64_t)k * b + i : zero);
+            load_row<S, NT>(X + xr * W + col0 + i * NT, bv);
+            // the next step's table entries (in range: 16 spare entries)
+            kb += 4;
+            nx0 = xrow[kb];
+            nx1 = xrow[kb + 1];
+            nx2 = xrow[kb + 2];
+            nx3 = xrow[kb + 3];
+            ++fst;
         };
-        for (int st = 0; st < nsteps; ++st) {
-            S a, bv[NT];
-            fetch(blk, kk, a, bv);
+        auto mma = [&](S a, const S (&bv)[NT]) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
-            kk += 4;
-            while (kk >= b) { kk -= b; ++blk; }
+        };
+        // The fetches are unconditional (the last pair fetches one step past
+        // the row: its A lanes read the zero word and its X rows are valid),
+        // so the wait before each MFMA group counts only the other buffer's
+        // loads on every path.
+        if (nsteps > 0) {
+            S a0, b0[NT], a1, b1[NT];
+            fetch(a0, b0);
+            int st = 0;
+            for (; st + 1 < nsteps; st += 2) {
+                fetch(a1, b1);
+                mma(a0, b0);
+                fetch(a0, b0);
+                mma(a1, b1);
+            }
+            if (st < nsteps) mma(a0, b0);
         }
 
         mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
@@ -855,23 +880,23 @@ hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const 
     switch (e.mode) {
     case EPI_PLAIN:
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
-                           A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+                           A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     case EPI_FIRST:
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
-                           A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+                           A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     case EPI_STEP:
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
-                           A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+                           A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     case EPI_PLAINSTEP:
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV>), gd, bd, 0, st, A.nbr,
-                           (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+                           (int)A.b, A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     default:
         hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV>), gd, bd, 0, st, A.nbr,
-                           (int)A.b, A.row_ptr, A.col_ind, vals, x, y, W, R, xcd, e);
+                           (int)A.b, A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     }
     return hipGetLastError();
@@ -929,9 +954,9 @@ hipError_t dma_by_nt(int nt, const BsrDev &A, const void *X, void *Y, int W, int
 }
 
 template <typename S, bool CV>
-hipError_t by_nt(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+hipError_t by_nt(int nt, const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                  int grid, hipStream_t st) {
-    switch (mfma_nt(W)) {
+    switch (nt) {
     case 2: return go_mfma<S, 2, CV>(A, X, Y, W, R, e, grid, st);
     case 4: return go_mfma<S, 4, CV>(A, X, Y, W, R, e, grid, st);
     case 8: return go_mfma<S, 8, CV>(A, X, Y, W, R, e, grid, st);
@@ -1374,14 +1399,14 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
     int nt = mfma_nt(W);
     while (nt > 2 && nt > g_mfma_ntcap) nt /= 2;
     const int nchunks = nt > 0 ? W / (16 * nt) : 0;
-    if (!A.is_complex && A.b >= 8 && nt > 0 && kWaves % nchunks == 0) {
+    if (!A.is_complex && A.b >= 8 && A.xrow && nt > 0 && kWaves % nchunks == 0) {
         int64_t g = (A.nbr * nchunks + kWaves - 1) / kWaves;
         if (g > kMaxGrid) g = kMaxGrid;
         g = ((g + 7) / 8) * 8; // whole XCD rounds (kMaxGrid is a multiple of 8)
         const int grid = (int)g;
         if (grid_out) *grid_out = grid;
-        if (g_dma_depth > 0 && A.xrow && nt * (vm.fp32 ? 4 : 8) >= 16 &&
-            (vm.fp32 || e.mode != EPI_PLAIN)) {
+        if (g_dma_depth > 0 && A.xrow && nt * (vm.fp32 ? 4 : 8) >= 16 && vm.fp32 &&
+            e.mode != EPI_PLAIN) {
             if (vm.fp32)
                 return vm.complex_vec ? dma_by_nt<float, true>(nt, A, X, Y, W, R, e, grid, stream)
                                       : dma_by_nt<float, false>(nt, A, X, Y, W, R, e, grid, stream);
@@ -1389,11 +1414,11 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
                                   : dma_by_nt<double, false>(nt, A, X, Y, W, R, e, grid, stream);
         }
         if (vm.fp32) {
-            return vm.complex_vec ? by_nt<float, true>(A, X, Y, W, R, e, grid, stream)
-                                  : by_nt<float, false>(A, X, Y, W, R, e, grid, stream);
+            return vm.complex_vec ? by_nt<float, true>(nt, A, X, Y, W, R, e, grid, stream)
+                                  : by_nt<float, false>(nt, A, X, Y, W, R, e, grid, stream);
         }
-        return vm.complex_vec ? by_nt<double, true>(A, X, Y, W, R, e, grid, stream)
-                              : by_nt<double, false>(A, X, Y, W, R, e, grid, stream);
+        return vm.complex_vec ? by_nt<double, true>(nt, A, X, Y, W, R, e, grid, stream)
+                              : by_nt<double, false>(nt, A, X, Y, W, R, e, grid, stream);
     }
     const int grid = spmm_grid(A.nbr, R);
     if (grid_out) *grid_out = grid;
