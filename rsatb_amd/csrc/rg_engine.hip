@@ -202,14 +202,14 @@ double matrix_flops(const Slot &s, VecMode vm, int R) {
 }
 
 // Gather table (see BsrDev::xrow) of a real slot with b >= 8, the blocks the
-// MFMA SpMMs take; 16 trailing entries keep the steps the kernels fetch past
-// the last row's K (at most 11 entries) in range.
+// MFMA SpMMs take; 32 trailing entries keep the steps the kernels fetch past
+// the last row's K (at most 30 entries) in range.
 int build_xrow(rg_context *ctx, Slot &s, const int32_t *ci) {
     s.has_xrow = false;
     if (s.b < 8 || s.is_complex) return RG_OK;
     const int64_t n = s.nnzb * s.b;
     if ((int64_t)s.nbr * s.b > INT32_MAX) return RG_OK;
-    std::vector<int32_t> t((size_t)n + 16, 0);
+    std::vector<int32_t> t((size_t)n + 32, 0);
     for (int64_t q = 0; q < s.nnzb; ++q)
         for (int kk = 0; kk < s.b; ++kk) t[(size_t)(q * s.b + kk)] = ci[q] * s.b + kk;
     RG_HIP(ctx, s.xrow.ensure(t.size() * 4), "hipMalloc gather table");
@@ -665,6 +665,8 @@ int rg_create(int device_id, rg_context **out) {
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
     if (const char *v = std::getenv("RG_UNION")) set_union_stage(std::atoi(v));
     if (const char *v = std::getenv("RG_DMA")) set_dma_depth(std::atoi(v));
+    if (const char *v = std::getenv("RG_MFMA_PD32")) set_mfma_pd32(std::atoi(v));
+    if (const char *v = std::getenv("RG_MFMA_PD64")) set_mfma_pd64(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RG_ERR_HIP;

@@ -23,7 +23,7 @@ struct BsrDev {
     const int32_t *grp_cols = nullptr;
     const int32_t *pos = nullptr;
     // gather table of real b >= 8 matrices, or null: xrow[s * b + kk] =
-    // col_ind[s] * b + kk, the X row of column kk of block s (16 spare entries)
+    // col_ind[s] * b + kk, the X row of column kk of block s (32 spare entries)
     const int32_t *xrow = nullptr;
 };
 
@@ -71,6 +71,8 @@ void set_xcd_map(int on);
 void set_mfma_ntcap(int nt);
 void set_union_stage(int on);
 void set_dma_depth(int p);
+void set_mfma_pd32(int p);
+void set_mfma_pd64(int p);
 int union_group_rows();
 
 // Device matrices hold every b x b block column-major; uploads arrive

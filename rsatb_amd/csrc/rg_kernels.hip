@@ -37,14 +37,21 @@ constexpr int kMaxGrid = 2048;
 int g_xcd_map = 1;
 // Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
 int g_mfma_ntcap = 16;
+// Register prefetch depths of the MFMA SpMM: FP64 plain products 3 (2 when
+// set to 2; steps always 2: their epilogue needs the registers), FP32 3 or 4.
+// Measured at the diamond 8^3 sandwich, R = 64 complex: FP64 plain 105.6 us
+// at 3 against 107.4 at 2, FP64 step 114.9 at 2 against 116.4 at 3; FP32
+// plain 54.0 and step 62.2 at 3, 70.8 at 4.
+int g_mfma_pd64 = 3;
+int g_mfma_pd32 = 3;
 // Union-staged path for dense blocks (when the matrix carries group lists).
 int g_union_stage = 0;
-// LDS-DMA staging depth P of the MFMA SpMM (k_spmm_dma); 0 keeps the
-// register-operand kernel k_spmm_mfma everywhere. With P > 0 the DMA kernel
-// runs the FP32 launches with a recurrence epilogue; the rest stay on
-// k_spmm_mfma, faster there (diamond 8^3 sandwich, R = 64 complex: FP64 plain
-// 113 us vs 140, FP64 step 142 vs 163, FP32 plain 62 vs 65; FP32 step 85 vs 92).
-int g_dma_depth = 2;
+// LDS-DMA staging depth P of the MFMA SpMM (k_spmm_dma); 0 (the default)
+// keeps the register kernel k_spmm_mfma everywhere. With P > 0 the DMA kernel
+// runs the FP32 launches with a recurrence epilogue. Since k_spmm_mfma
+// prefetches in registers it is the faster of the two at the diamond 8^3
+// sandwich (FP32 step 62 us against 81 us); the DMA kernel stays selectable.
+int g_dma_depth = 0;
 // LDS bytes one union-staged workgroup stages per chunk (two groups per CU).
 constexpr size_t kGrpLdsBudget = 72 * 1024;
 
@@ -486,7 +493,7 @@ __device__ __forceinline__ void mfma_dot_partials(
     }
 }
 
-template <typename S, int NT, int MODE, bool CV>
+template <typename S, int NT, int MODE, bool CV, int PD>
 __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                                                    const int64_t *__restrict__ row_ptr,
                                                    const int32_t *__restrict__ xrow,
@@ -555,7 +562,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
             const int k = 4 * fst + g;
             a = __builtin_nontemporal_load((i < b && k < K) ? panel + (int64_t)k * b + i : zero);
             load_row<S, NT>(X + xr * W + col0 + i * NT, bv);
-            // the next step's table entries (in range: 16 spare entries)
+            // the next step's table entries (in range: 32 spare entries)
             kb += 4;
             nx0 = xrow[kb];
             nx1 = xrow[kb + 1];
@@ -567,21 +574,21 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
 #pragma unroll
             for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
         };
-        // The fetches are unconditional (the last pair fetches one step past
-        // the row: its A lanes read the zero word and its X rows are valid),
-        // so the wait before each MFMA group counts only the other buffer's
-        // loads on every path.
+        // PD register buffers, PD - 1 steps in flight. The fetches are
+        // unconditional (up to 2 PD - 2 steps past the row: their A lanes read
+        // the zero word and their X rows are valid), so the wait before each
+        // MFMA group counts only the later buffers' loads on every path.
         if (nsteps > 0) {
-            S a0, b0[NT], a1, b1[NT];
-            fetch(a0, b0);
-            int st = 0;
-            for (; st + 1 < nsteps; st += 2) {
-                fetch(a1, b1);
-                mma(a0, b0);
-                fetch(a0, b0);
-                mma(a1, b1);
+            S ab[PD], bb[PD][NT];
+#pragma unroll
+            for (int q = 0; q < PD - 1; ++q) fetch(ab[q], bb[q]);
+            for (int st = 0; st < nsteps; st += PD) {
+#pragma unroll
+                for (int q = 0; q < PD; ++q) {
+                    fetch(ab[(q + PD - 1) % PD], bb[(q + PD - 1) % PD]);
+                    if (st + q < nsteps) mma(ab[q], bb[q]);
+                }
             }
-            if (st < nsteps) mma(a0, b0);
         }
 
         mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
@@ -869,8 +876,8 @@ __global__ __launch_bounds__(256) void k_spmm_dma(int64_t nbr, int b,
     mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
 }
 
-template <typename S, int NT, bool CV>
-hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+template <typename S, int NT, bool CV, int PD>
+hipError_t go_mfma_pd(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                       int grid, hipStream_t st) {
     const int xcd = g_xcd_map;
     const dim3 gd(grid), bd(64 * kWaves);
@@ -879,23 +886,23 @@ hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const 
     S *y = static_cast<S *>(Y);
     switch (e.mode) {
     case EPI_PLAIN:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV, PD>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     case EPI_FIRST:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV, PD>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     case EPI_STEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV>), gd, bd, 0, st, A.nbr, (int)A.b,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, PD>), gd, bd, 0, st, A.nbr, (int)A.b,
                            A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     case EPI_PLAINSTEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV>), gd, bd, 0, st, A.nbr,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, PD>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     default:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV>), gd, bd, 0, st, A.nbr,
+        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV, PD>), gd, bd, 0, st, A.nbr,
                            (int)A.b, A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
         break;
     }
@@ -950,6 +957,20 @@ hipError_t dma_by_nt(int nt, const BsrDev &A, const void *X, void *Y, int W, int
     case 4: return go_dma<S, 4, CV>(A, X, Y, W, R, e, grid, st);
     case 8: return go_dma<S, 8, CV>(A, X, Y, W, R, e, grid, st);
     default: return go_dma<S, 16, CV>(A, X, Y, W, R, e, grid, st);
+    }
+}
+
+// register prefetch depth of k_spmm_mfma (g_mfma_pd64, g_mfma_pd32)
+template <typename S, int NT, bool CV>
+hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
+                   int grid, hipStream_t st) {
+    if constexpr (sizeof(S) == 8) {
+        if (g_mfma_pd64 == 3 && e.mode == EPI_PLAIN)
+            return go_mfma_pd<S, NT, CV, 3>(A, X, Y, W, R, e, grid, st);
+        return go_mfma_pd<S, NT, CV, 2>(A, X, Y, W, R, e, grid, st);
+    } else {
+        if (g_mfma_pd32 == 4) return go_mfma_pd<S, NT, CV, 4>(A, X, Y, W, R, e, grid, st);
+        return go_mfma_pd<S, NT, CV, 3>(A, X, Y, W, R, e, grid, st);
     }
 }
 
@@ -1437,6 +1458,8 @@ int spmm_max_grid() { return kMaxGrid; }
 void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_union_stage(int on) { g_union_stage = on ? 1 : 0; }
+void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
+void set_mfma_pd64(int p) { g_mfma_pd64 = p == 2 ? 2 : 3; }
 void set_dma_depth(int p) { g_dma_depth = (p == 0 || p == 4) ? p : 2; }
 int union_group_rows() { return kGrpWaves; }
 
