@@ -181,9 +181,12 @@ def test_graphene_moments_and_dos_match_oracle(gpu_ctx, oracle):
     assert rel_err(dos, rdos) < 1e-10
 
 
-def test_diamond_sandwich_moments_match_oracle(gpu_ctx, oracle):
+@pytest.mark.parametrize("R", [16, 64, 128])
+def test_diamond_sandwich_moments_match_oracle(gpu_ctx, oracle, R):
     # diamond 2x2x2 conventional (64 atoms, 13 orbitals), Z from the oracle's
-    # patches, R = 64 random phase, doubled and plain
+    # patches, random phase, doubled and plain; R = 16, 64, 128 take the VALU
+    # kernel and the MFMA kernels with 8 and 16 column tiles per wave. The
+    # precision form follows the same moments to the FP32 bound.
     st = diamond_supercell(2)
     e = neighbor_edges(st, 2.7)
     m = RandomBlockModel()
@@ -200,14 +203,17 @@ def test_diamond_sandwich_moments_match_oracle(gpu_ctx, oracle):
     gpu_ctx.upload_bsr(_lib.SLOT_ZH, Zb.conj_transpose())
     gpu_ctx.set_operator(_lib.FORM_SANDWICH, c, a)
     rows = st.n_atoms * 13
-    gpu_ctx.set_samples(rows, 64, _lib.SAMPLES_PHASE, 20240917)
+    gpu_ctx.set_samples(rows, R, _lib.SAMPLES_PHASE, 20240917)
     psi = gpu_ctx.get_samples()
     op = oracle.Operator(1, H, c, a, Z)
     M = 96
     for dbl in (True, False):
-        mu = gpu_ctx.kpm_moments(M, doubling=_lib.DOUBLING_ON if dbl else _lib.DOUBLING_OFF)
+        d = _lib.DOUBLING_ON if dbl else _lib.DOUBLING_OFF
+        mu = gpu_ctx.kpm_moments(M, doubling=d)
         ref = op.moments(psi, M, doubling=dbl)
         assert np.max(np.abs(mu - ref)) < 1e-12, dbl
+        mu32 = gpu_ctx.kpm_moments(M, _lib.PREC_FP32, d)
+        assert np.max(np.abs(mu32 - ref)) < 2e-5, dbl
     assert np.all(np.abs(mu) <= 1.0 + 1e-12)
 
 
