@@ -23,12 +23,6 @@ using namespace rg;
 
 namespace {
 
-// Bytes allocated past the end of every matrix value array: the MFMA SpMMs
-// read A fragments up to 8 K columns past a row's panel (the DMA kernel moves
-// 512 bytes from a step's first column; the register kernel prefetches one
-// step ahead); past the last block row those reads leave the values proper,
-// their lanes are masked, and the bytes must exist.
-constexpr size_t kValsPad = 2048;
 
 struct DevBuf {
     void *p = nullptr;
@@ -61,10 +55,7 @@ struct Slot {
     bool is_complex = false;
     DevBuf row_ptr, col_ind, vals64, vals32;
     bool vals32_valid = false;
-    // union lists for the union-staged SpMM (dense blocks, sorted rows)
-    DevBuf grp_ptr, grp_cols, pos;
-    bool has_groups = false;
-    // gather table of the DMA-staged SpMM: xrow[s * b + kk] = col_ind[s] * b + kk
+    // gather table of the MFMA SpMM: xrow[s * b + kk] = col_ind[s] * b + kk
     DevBuf xrow;
     bool has_xrow = false;
 };
@@ -183,11 +174,6 @@ BsrDev dev_view(const Slot &s, bool fp32) {
     d.vals = fp32 ? s.vals32.p : s.vals64.p;
     d.is_complex = s.is_complex;
     if (s.has_xrow) d.xrow = static_cast<const int32_t *>(s.xrow.p);
-    if (s.has_groups) {
-        d.grp_ptr = static_cast<const int64_t *>(s.grp_ptr.p);
-        d.grp_cols = static_cast<const int32_t *>(s.grp_cols.p);
-        d.pos = static_cast<const int32_t *>(s.pos.p);
-    }
     return d;
 }
 
@@ -202,8 +188,8 @@ double matrix_flops(const Slot &s, VecMode vm, int R) {
 }
 
 // Gather table (see BsrDev::xrow) of a real slot with b >= 8, the blocks the
-// MFMA SpMMs take; 32 trailing entries keep the steps the kernels fetch past
-// the last row's K (at most 30 entries) in range.
+// MFMA SpMM takes; 32 trailing entries keep the steps it fetches past the last
+// row's K (at most 30 entries) in range.
 int build_xrow(rg_context *ctx, Slot &s, const int32_t *ci) {
     s.has_xrow = false;
     if (s.b < 8 || s.is_complex) return RG_OK;
@@ -215,41 +201,6 @@ int build_xrow(rg_context *ctx, Slot &s, const int32_t *ci) {
     RG_HIP(ctx, s.xrow.ensure(t.size() * 4), "hipMalloc gather table");
     RG_HIP(ctx, hipMemcpy(s.xrow.p, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload gather table");
     s.has_xrow = true;
-    return RG_OK;
-}
-
-// Union lists of consecutive groups of block rows (see BsrDev) for a slot
-// with dense blocks and sorted rows; other slots go without.
-int build_groups(rg_context *ctx, Slot &s, const int64_t *rp, const int32_t *ci) {
-    s.has_groups = false;
-    if (s.b < 8 || s.is_complex) return RG_OK;
-    for (int64_t r = 0; r < s.nbr; ++r)
-        for (int64_t q = rp[r] + 1; q < rp[r + 1]; ++q)
-            if (ci[q] <= ci[q - 1]) return RG_OK; // unsorted rows: no union path
-    const int G = union_group_rows();
-    const int64_t ng = (s.nbr + G - 1) / G;
-    std::vector<int64_t> gp((size_t)ng + 1, 0);
-    std::vector<int32_t> gc, pos((size_t)s.nnzb);
-    std::vector<int32_t> u;
-    for (int64_t gi = 0; gi < ng; ++gi) {
-        const int64_t r0 = gi * G, r1 = std::min(s.nbr, r0 + G);
-        u.assign(ci + rp[r0], ci + rp[r1]);
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        for (int64_t q = rp[r0]; q < rp[r1]; ++q)
-            pos[q] = (int32_t)(std::lower_bound(u.begin(), u.end(), ci[q]) - u.begin());
-        gc.insert(gc.end(), u.begin(), u.end());
-        gp[gi + 1] = (int64_t)gc.size();
-    }
-    RG_HIP(ctx, s.grp_ptr.ensure(gp.size() * 8), "hipMalloc groups");
-    RG_HIP(ctx, s.grp_cols.ensure(gc.size() * 4 + 4), "hipMalloc groups");
-    RG_HIP(ctx, s.pos.ensure(pos.size() * 4 + 4), "hipMalloc groups");
-    RG_HIP(ctx, hipMemcpy(s.grp_ptr.p, gp.data(), gp.size() * 8, hipMemcpyHostToDevice), "upload groups");
-    if (!gc.empty())
-        RG_HIP(ctx, hipMemcpy(s.grp_cols.p, gc.data(), gc.size() * 4, hipMemcpyHostToDevice), "upload groups");
-    if (!pos.empty())
-        RG_HIP(ctx, hipMemcpy(s.pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice), "upload groups");
-    s.has_groups = true;
     return RG_OK;
 }
 
@@ -384,7 +335,7 @@ int ensure_fp32(rg_context *ctx) {
         Slot &s = ctx->slots[slots[i]];
         if (s.vals32_valid) continue;
         const int64_t n = s.nnzb * s.b * s.b * (s.is_complex ? 2 : 1);
-        RG_HIP(ctx, s.vals32.ensure((size_t)n * 4 + kValsPad), "hipMalloc fp32 values");
+        RG_HIP(ctx, s.vals32.ensure((size_t)n * 4), "hipMalloc fp32 values");
         RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.vals64.p),
                                       static_cast<float *>(s.vals32.p), n, ctx->stream),
                "fp32 conversion");
@@ -663,8 +614,6 @@ int rg_create(int device_id, rg_context **out) {
     ctx->device = device_id;
     if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
-    if (const char *v = std::getenv("RG_UNION")) set_union_stage(std::atoi(v));
-    if (const char *v = std::getenv("RG_DMA")) set_dma_depth(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD32")) set_mfma_pd32(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD64")) set_mfma_pd64(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -689,9 +638,6 @@ void rg_destroy(rg_context *ctx) {
         s.col_ind.release();
         s.vals64.release();
         s.vals32.release();
-        s.grp_ptr.release();
-        s.grp_cols.release();
-        s.pos.release();
         s.xrow.release();
     }
     ctx->psi.release();
@@ -729,7 +675,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     const size_t nv = (size_t)nnzb * b * b * (dtype == RG_C128 ? 2 : 1);
     RG_HIP(ctx, s.row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc row_ptr");
     RG_HIP(ctx, s.col_ind.ensure((size_t)nnzb * 4 + 4), "hipMalloc col_ind");
-    RG_HIP(ctx, s.vals64.ensure(nv * 8 + kValsPad), "hipMalloc values");
+    RG_HIP(ctx, s.vals64.ensure(nv * 8), "hipMalloc values");
     RG_HIP(ctx, hipMemcpy(s.row_ptr.p, row_ptr, (size_t)(nbr + 1) * 8, hipMemcpyHostToDevice), "upload row_ptr");
     if (nnzb > 0) {
         RG_HIP(ctx, hipMemcpy(s.col_ind.p, col_ind, (size_t)nnzb * 4, hipMemcpyHostToDevice), "upload col_ind");
@@ -749,8 +695,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     s.b = b;
     s.nnzb = nnzb;
     s.is_complex = (dtype == RG_C128);
-    int rcg = build_groups(ctx, s, row_ptr, col_ind);
-    if (!rcg) rcg = build_xrow(ctx, s, col_ind);
+    int rcg = build_xrow(ctx, s, col_ind);
     if (rcg) return rcg;
     s.present = true;
     return RG_OK;
@@ -1048,7 +993,7 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         t->vals32_valid = false;
         RG_HIP(ctx, t->row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc");
         RG_HIP(ctx, t->col_ind.ensure((size_t)S.nnzb * 4 + 4), "hipMalloc");
-        RG_HIP(ctx, t->vals64.ensure((size_t)S.nnzb * b * b * 8 + kValsPad), "hipMalloc");
+        RG_HIP(ctx, t->vals64.ensure((size_t)S.nnzb * b * b * 8), "hipMalloc");
         RG_HIP(ctx, hipMemcpy(t->row_ptr.p, S.row_ptr.p, (size_t)(nbr + 1) * 8, hipMemcpyDeviceToDevice), "copy");
         if (S.nnzb)
             RG_HIP(ctx, hipMemcpy(t->col_ind.p, S.col_ind.p, (size_t)S.nnzb * 4, hipMemcpyDeviceToDevice), "copy");
@@ -1056,8 +1001,7 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         t->b = b;
         t->nnzb = S.nnzb;
         t->is_complex = false;
-        int rcg = build_groups(ctx, *t, rp.data(), ci.data());
-        if (!rcg) rcg = build_xrow(ctx, *t, ci.data());
+        int rcg = build_xrow(ctx, *t, ci.data());
         if (rcg) return rcg;
     }
     int it_max = 0;

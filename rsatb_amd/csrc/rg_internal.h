@@ -16,12 +16,6 @@ struct BsrDev {
     const int32_t *col_ind = nullptr;
     const void *vals = nullptr; // double / double2 / float / float2
     bool is_complex = false;
-    // union lists of consecutive groups of union_group_rows() block rows
-    // (group g's sorted distinct block columns are grp_cols[grp_ptr[g] ..
-    // grp_ptr[g+1]); pos[s] is block s's index in its group's list), or null
-    const int64_t *grp_ptr = nullptr;
-    const int32_t *grp_cols = nullptr;
-    const int32_t *pos = nullptr;
     // gather table of real b >= 8 matrices, or null: xrow[s * b + kk] =
     // col_ind[s] * b + kk, the X row of column kk of block s (32 spare entries)
     const int32_t *xrow = nullptr;
@@ -69,11 +63,8 @@ int spmm_max_grid();
 // XCD-contiguous row ranges in the MFMA SpMM (default on).
 void set_xcd_map(int on);
 void set_mfma_ntcap(int nt);
-void set_union_stage(int on);
-void set_dma_depth(int p);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
-int union_group_rows();
 
 // Device matrices hold every b x b block column-major; uploads arrive
 // row-major (rsatb's layout) and are transposed on the device.

@@ -44,16 +44,6 @@ int g_mfma_ntcap = 16;
 // plain 54.0 and step 62.2 at 3, 70.8 at 4.
 int g_mfma_pd64 = 3;
 int g_mfma_pd32 = 3;
-// Union-staged path for dense blocks (when the matrix carries group lists).
-int g_union_stage = 0;
-// LDS-DMA staging depth P of the MFMA SpMM (k_spmm_dma); 0 (the default)
-// keeps the register kernel k_spmm_mfma everywhere. With P > 0 the DMA kernel
-// runs the FP32 launches with a recurrence epilogue. Since k_spmm_mfma
-// prefetches in registers it is the faster of the two at the diamond 8^3
-// sandwich (FP32 step 62 us against 81 us); the DMA kernel stays selectable.
-int g_dma_depth = 0;
-// LDS bytes one union-staged workgroup stages per chunk (two groups per CU).
-constexpr size_t kGrpLdsBudget = 72 * 1024;
 
 // Tiles per wave of the MFMA path for W real columns (0: not eligible).
 __host__ __device__ inline int mfma_nt(int W) {
@@ -313,10 +303,10 @@ hipError_t by_block(const BsrDev &A, const void *X, void *Y, int R, const EpiArg
 // Operand layouts of v_mfma_{f64,f32}_16x16x4 (lane l, i = l % 16, g = l / 16):
 //   A[i][k = g], B[k = g][j = i]; accumulator element v holds D[4 g + v][i]
 //   for f32 and D[4 v + g][i] for f64 (both measured on gfx950).
-// N is relabelled so that lane i owns NT *contiguous* real columns
-// col0 + i NT + t (tile t): X, Y and the recurrence operands move as wide
-// vector loads/stores, and the (re, im) halves of a complex column sit in
-// the same lane (tiles 2u, 2u + 1), so the complex dots need no shuffles.
+// N is relabelled (col_of below) so that X, Y and the recurrence operands
+// move as coalesced 16-byte loads/stores, and the (re, im) halves of a
+// complex column sit in the same lane (tiles 2u, 2u + 1), so the complex
+// dots need no shuffles.
 // Blocks are stored column-major on the device, so the 13 rows of one A
 // column are contiguous and the A fragment load is coalesced.
 template <typename S> struct Acc4;
@@ -330,42 +320,63 @@ __device__ __forceinline__ Acc4<float>::T mfma16(float a, float b, Acc4<float>::
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// NT contiguous elements; NT * sizeof(S) is a multiple of 8 bytes.
-template <typename S, int NT>
-__device__ __forceinline__ void load_row(const S *__restrict__ p, S (&v)[NT]) {
-    if constexpr ((NT * sizeof(S)) % 16 == 0) {
-        const float4 *q = reinterpret_cast<const float4 *>(p);
-        constexpr int NQ = NT * sizeof(S) / 16;
-#pragma unroll
-        for (int t = 0; t < NQ; ++t) {
-            const float4 w = q[t];
-            __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * t, &w, 16);
-        }
-    } else {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) v[t] = p[t];
-    }
+// Column mapping of the MFMA SpMM: tile t of lane i holds real column
+//   col(t, i) = (t / E) 16 E + i E + t % E,   E = min(NT, 16 / sizeof(S)),
+// so every 16-byte load (E elements) of a row's 16 lanes covers 16 E
+// contiguous columns, and the (re, im) halves of a complex column (E is even)
+// stay in one lane, next to each other in t.
+template <typename S, int NT> __host__ __device__ constexpr int col_epl() {
+    return NT < 16 / (int)sizeof(S) ? NT : 16 / (int)sizeof(S);
 }
+template <typename S, int NT> __device__ __forceinline__ int col_of(int t, int i) {
+    constexpr int E = col_epl<S, NT>();
+    return (t / E) * 16 * E + i * E + t % E;
+}
+
+// v[t] = row[col_of(t, i)]
 template <typename S, int NT>
-__device__ __forceinline__ void store_row(S *__restrict__ p, const S (&v)[NT]) {
-    if constexpr ((NT * sizeof(S)) % 16 == 0) {
-        float4 *q = reinterpret_cast<float4 *>(p);
-        constexpr int NQ = NT * sizeof(S) / 16;
+__device__ __forceinline__ void load_cols(const S *__restrict__ row, int i, S (&v)[NT]) {
+    constexpr int E = col_epl<S, NT>();
+    constexpr int B = E * (int)sizeof(S);
 #pragma unroll
-        for (int t = 0; t < NQ; ++t) {
-            float4 w;
-            __builtin_memcpy(&w, reinterpret_cast<const char *>(v) + 16 * t, 16);
-            q[t] = w;
+    for (int q = 0; q < NT / E; ++q) {
+        const S *p = row + q * 16 * E + i * E;
+        if constexpr (B == 16) {
+            const float4 w = *reinterpret_cast<const float4 *>(p);
+            __builtin_memcpy(&v[q * E], &w, 16);
+        } else if constexpr (B == 8) {
+            const float2 w = *reinterpret_cast<const float2 *>(p);
+            __builtin_memcpy(&v[q * E], &w, 8);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[q * E + e] = p[e];
         }
-    } else {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) p[t] = v[t];
     }
 }
 
-// Epilogue of one finished 16-row tile of the MFMA SpMMs: the rows r < b of
-// block row br, columns col0 + i NT + t, from the accumulators; recurrence,
-// dots and store as EpiArgs says.
+// row[col_of(t, i)] = v[t]
+template <typename S, int NT>
+__device__ __forceinline__ void store_cols(S *__restrict__ row, int i, const S (&v)[NT]) {
+    constexpr int E = col_epl<S, NT>();
+    constexpr int B = E * (int)sizeof(S);
+#pragma unroll
+    for (int q = 0; q < NT / E; ++q) {
+        S *p = row + q * 16 * E + i * E;
+        if constexpr (B == 16) {
+            float4 w;
+            __builtin_memcpy(&w, &v[q * E], 16);
+            *reinterpret_cast<float4 *>(p) = w;
+        } else if constexpr (B == 8) {
+            float2 w;
+            __builtin_memcpy(&w, &v[q * E], 8);
+            *reinterpret_cast<float2 *>(p) = w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) p[e] = v[q * E + e];
+        }
+    }
+}
+
 template <typename S, int NT, int MODE, bool CV>
 __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[NT], int64_t br,
                                               int b, int W, int col0, int i, int g,
@@ -382,7 +393,7 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
         // accumulator element v holds row 4 v + g (f64) or 4 g + v (f32)
         const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
         if (r >= b) continue;
-        const int64_t base = (br * b + r) * (int64_t)W + col0 + i * NT;
+        const int64_t base = (br * b + r) * (int64_t)W + col0;
         S y[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -391,13 +402,13 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
         }
         if constexpr (MODE != EPI_PLAIN) {
             S xs[NT];
-            load_row<S, NT>(XS + base, xs);
+            load_cols<S, NT>(XS + base, i, xs);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
             if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
                 S xp[NT];
-                load_row<S, NT>(XP + base, xp);
+                load_cols<S, NT>(XP + base, i, xp);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
             }
@@ -408,7 +419,7 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     S qv[NT];
-                    load_row<S, NT>(qs[q] + base, qv);
+                    load_cols<S, NT>(qs[q] + base, i, qv);
 #pragma unroll
                     for (int t = 0; t < NT; ++t)
                         y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
@@ -440,11 +451,11 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
                 dot(1, y, xs, false);
             } else if constexpr (MODE == EPI_PLAINSTEP) {
                 S ps[NT];
-                load_row<S, NT>(PSI + base, ps);
+                load_cols<S, NT>(PSI + base, i, ps);
                 dot(0, ps, y, false);
             }
         }
-        store_row<S, NT>(Y + base, y);
+        store_cols<S, NT>(Y + base, i, y);
     }
 }
 
@@ -473,7 +484,7 @@ __device__ __forceinline__ void mfma_dot_partials(
             if (g == 0) {
 #pragma unroll
                 for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
-                    const int uu = CV ? (i * NT + t) / 2 : i * NT + t;
+                    const int uu = CV ? col_of<S, NT>(t, i) / 2 : col_of<S, NT>(t, i);
                     sm[wave][uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
                 }
             }
@@ -561,7 +572,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
             const int64_t xr = g == 0 ? nx0 : g == 1 ? nx1 : g == 2 ? nx2 : nx3;
             const int k = 4 * fst + g;
             a = __builtin_nontemporal_load((i < b && k < K) ? panel + (int64_t)k * b + i : zero);
-            load_row<S, NT>(X + xr * W + col0 + i * NT, bv);
+            load_cols<S, NT>(X + xr * W + col0, i, bv);
             // the next step's table entries (in range: 32 spare entries)
             kb += 4;
             nx0 = xrow[kb];
@@ -594,285 +605,6 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
         mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
     }
 
-    mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
-}
-
-// ------------------------------------------------ DMA-staged MFMA SpMM
-// The arithmetic of k_spmm_mfma, with the operands of every 4-wide K step
-// (the four gathered X row chunks and the wave's 16 x 4 A fragment) moved
-// global -> LDS by the LDS-DMA path (global_load_lds, no VGPRs held while in
-// flight), P - 1 steps ahead of the MFMAs that consume them. The stream of
-// steps runs across the wave's block rows, so the next row's first steps are
-// in flight during a row's epilogue. Each wave owns P step buffers of its own
-// LDS; only vmcnt orders a buffer's DMA before its reads, and a buffer is
-// refilled one step after it was read (the read waits on lgkmcnt), so no
-// barrier is needed.
-
-// s_waitcnt vmcnt(VM), then N16 16-byte B chunks (at xaddr + 256 q) and the A
-// element (at aaddr, AB bytes) from LDS; the LDS reads complete (lgkmcnt(0))
-// before the asm ends, so its outputs are ready when it returns.
-using F4 = float __attribute__((ext_vector_type(4)));
-template <int VM, int N16, int AB> struct DmaRead;
-template <int VM> struct DmaRead<VM, 1, 8> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[1], double &a) {
-        asm volatile("s_waitcnt vmcnt(%3)\n\t"
-                     "ds_read_b128 %0, %2\n\t"
-                     "ds_read_b64 %1, %4\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 2, 8> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[2], double &a) {
-        asm volatile("s_waitcnt vmcnt(%4)\n\t"
-                     "ds_read_b128 %0, %3\n\t"
-                     "ds_read_b128 %1, %3 offset:256\n\t"
-                     "ds_read_b64 %2, %5\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 4, 8> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[4], double &a) {
-        asm volatile("s_waitcnt vmcnt(%6)\n\t"
-                     "ds_read_b128 %0, %5\n\t"
-                     "ds_read_b128 %1, %5 offset:256\n\t"
-                     "ds_read_b128 %2, %5 offset:512\n\t"
-                     "ds_read_b128 %3, %5 offset:768\n\t"
-                     "ds_read_b64 %4, %7\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 8, 8> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[8], double &a) {
-        asm volatile("s_waitcnt vmcnt(%10)\n\t"
-                     "ds_read_b128 %0, %9\n\t"
-                     "ds_read_b128 %1, %9 offset:256\n\t"
-                     "ds_read_b128 %2, %9 offset:512\n\t"
-                     "ds_read_b128 %3, %9 offset:768\n\t"
-                     "ds_read_b128 %4, %9 offset:1024\n\t"
-                     "ds_read_b128 %5, %9 offset:1280\n\t"
-                     "ds_read_b128 %6, %9 offset:1536\n\t"
-                     "ds_read_b128 %7, %9 offset:1792\n\t"
-                     "ds_read_b64 %8, %11\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 1, 4> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[1], float &a) {
-        asm volatile("s_waitcnt vmcnt(%3)\n\t"
-                     "ds_read_b128 %0, %2\n\t"
-                     "ds_read_b32 %1, %4\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 2, 4> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[2], float &a) {
-        asm volatile("s_waitcnt vmcnt(%4)\n\t"
-                     "ds_read_b128 %0, %3\n\t"
-                     "ds_read_b128 %1, %3 offset:256\n\t"
-                     "ds_read_b32 %2, %5\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 4, 4> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[4], float &a) {
-        asm volatile("s_waitcnt vmcnt(%6)\n\t"
-                     "ds_read_b128 %0, %5\n\t"
-                     "ds_read_b128 %1, %5 offset:256\n\t"
-                     "ds_read_b128 %2, %5 offset:512\n\t"
-                     "ds_read_b128 %3, %5 offset:768\n\t"
-                     "ds_read_b32 %4, %7\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-template <int VM> struct DmaRead<VM, 8, 4> {
-    __device__ static __forceinline__ void run(unsigned xaddr, unsigned aaddr, F4 (&o)[8], float &a) {
-        asm volatile("s_waitcnt vmcnt(%10)\n\t"
-                     "ds_read_b128 %0, %9\n\t"
-                     "ds_read_b128 %1, %9 offset:256\n\t"
-                     "ds_read_b128 %2, %9 offset:512\n\t"
-                     "ds_read_b128 %3, %9 offset:768\n\t"
-                     "ds_read_b128 %4, %9 offset:1024\n\t"
-                     "ds_read_b128 %5, %9 offset:1280\n\t"
-                     "ds_read_b128 %6, %9 offset:1536\n\t"
-                     "ds_read_b128 %7, %9 offset:1792\n\t"
-                     "ds_read_b32 %8, %11\n\t"
-                     "s_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(a)
-                     : "v"(xaddr), "i"(VM), "v"(aaddr)
-                     : "memory");
-    }
-};
-
-// 32-bit LDS byte address of a pointer into __shared__ memory
-__device__ __forceinline__ unsigned lds_addr(const void *p) {
-    return (unsigned)(uintptr_t)((const __attribute__((address_space(3))) char *)p);
-}
-#define RG_LDS(p) ((__attribute__((address_space(3))) void *)(p))
-
-template <typename S, int NT>
-__host__ __device__ constexpr int dma_step_bytes() {
-    return 4 * 16 * NT * (int)sizeof(S) + 64 * (int)sizeof(S);
-}
-
-template <typename S, int NT, int MODE, bool CV, int P>
-__global__ __launch_bounds__(256) void k_spmm_dma(int64_t nbr, int b,
-                                                  const int64_t *__restrict__ row_ptr,
-                                                  const int32_t *__restrict__ xrow,
-                                                  const S *__restrict__ vals,
-                                                  const S *__restrict__ X, S *__restrict__ Y,
-                                                  int W, int R, int xcd_map, EpiArgs e) {
-    using A4 = typename Acc4<S>::T;
-    constexpr int NDOT = epi_ndot(MODE);
-    constexpr int CW = 16 * NT;
-    constexpr int ROWB = CW * (int)sizeof(S); // bytes of one gathered X row chunk
-    constexpr int XB = 4 * ROWB;              // X bytes per step
-    static_assert(XB % 1024 == 0, "one X DMA instruction moves 1 KB");
-    constexpr int NI = XB / 1024;             // X DMA instructions per step
-    constexpr int NA = (int)sizeof(S) / 4;    // A DMA instructions per step (256 B each)
-    constexpr int STB = dma_step_bytes<S, NT>();
-    constexpr int VM = NI + NA;               // vmcnt increments per step
-    static_assert(VM * (P - 1) <= 63, "vmcnt holds 6 bits");
-    constexpr int N16 = NT * (int)sizeof(S) / 16;
-    static_assert(N16 >= 1, "B chunk of at least 16 bytes");
-    extern __shared__ __attribute__((aligned(16))) unsigned char dma_lds[];
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = lane & 15;
-    const int g = lane >> 4;
-    const int nchunks = W / CW; // 1, 2 or 4 (divides kWaves): a wave keeps chunk wave % nchunks
-    const int cshift = nchunks == 4 ? 2 : nchunks == 2 ? 1 : 0;
-    const int col0 = (wave % nchunks) * CW;
-    int64_t r_lo = 0, r_hi = nbr;
-    int64_t kq = blockIdx.x, nk = gridDim.x;
-    if (xcd_map) { // as k_spmm_mfma: each XCD walks its own eighth of the rows
-        const int x = blockIdx.x & 7;
-        kq = blockIdx.x >> 3;
-        nk = gridDim.x >> 3;
-        r_lo = nbr * x / 8;
-        r_hi = nbr * (x + 1) / 8;
-    }
-    const int64_t total = (r_hi - r_lo) * nchunks;
-    const int64_t wstride = nk * kWaves;
-    unsigned char *wbuf = dma_lds + wave * (P * STB);
-    const unsigned wbuf_addr = lds_addr(wbuf);
-    const char *Xc = reinterpret_cast<const char *>(X) + (size_t)col0 * sizeof(S);
-    const char *Vc = reinterpret_cast<const char *>(vals);
-    // LDS order of a row chunk: 16-byte chunk c = i N16 + q of the row (lane
-    // i's q-th chunk) sits at position q 16 + i, so each of a lane group's N16
-    // ds_read_b128 touches 256 contiguous bytes (no bank conflicts). The DMA
-    // lane filling position pos of the step's X region therefore fetches row
-    // pos / (16 N16), chunk ((pp & 15) N16 + pp / 16) of it (pp: position in
-    // the row).
-
-    // ---- issue side: the wave's K steps in consumption order. Within a
-    // block row the blocks are consecutive and column-major, so the row's A
-    // panel is one column-major b x K matrix: step st's four columns are the
-    // 4 b contiguous elements from (s0 b + 4 st) b. The step's X rows come
-    // from the gather table at s0 b + 4 st.
-    int64_t iit = kq * kWaves + wave; // work item whose steps are being issued
-    int64_t is0 = 0;                  // its first block
-    int insteps = 0, ist = 0;
-    auto open_row = [&]() { // from iit, the next work item with at least one step
-        while (iit < total) {
-            const int64_t br = r_lo + (iit >> cshift);
-            is0 = row_ptr[br];
-            insteps = ((int)(row_ptr[br + 1] - is0) * b + 3) >> 2;
-            if (insteps > 0) break;
-            iit += wstride;
-        }
-        ist = 0;
-    };
-    open_row();
-    // DMAs the next step into buffer `slot`; false once the stream is done
-    auto issue = [&](int slot) -> bool {
-        if (iit >= total) return false;
-        unsigned char *buf = wbuf + slot * STB;
-        const int64_t kb = is0 * b + 4 * ist;
-        int64_t xr[4];
-#pragma unroll
-        for (int kr = 0; kr < 4; ++kr) xr[kr] = (int64_t)xrow[kb + kr] * W;
-#pragma unroll
-        for (int j = 0; j < NI; ++j) {
-            // instruction j fills positions [64 j, 64 j + 64) of the step
-            const int pos = j * 64 + lane;
-            const int kr = pos / (16 * N16);
-            const int pp = pos - kr * (16 * N16);
-            const int src = ((pp & 15) * N16 + (pp >> 4)) * 16;
-            int64_t r;
-            if constexpr (ROWB >= 1024) r = xr[(j * 1024) / ROWB];
-            else r = kr == 0 ? xr[0] : kr == 1 ? xr[1] : kr == 2 ? xr[2] : xr[3];
-            __builtin_amdgcn_global_load_lds(Xc + r * (int64_t)sizeof(S) + src,
-                                             RG_LDS(buf + j * 1024), 16, 0, 0);
-        }
-        const char *asrc = Vc + (kb * b) * (int64_t)sizeof(S) + 4 * lane;
-#pragma unroll
-        for (int q = 0; q < NA; ++q)
-            __builtin_amdgcn_global_load_lds(asrc + 256 * q, RG_LDS(buf + XB + 256 * q), 4, 0, 0);
-        if (++ist == insteps) {
-            iit += wstride;
-            open_row();
-        }
-        return true;
-    };
-
-    double dacc[NDOT > 0 ? NDOT : 1][NT];
-#pragma unroll
-    for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
-
-#pragma unroll
-    for (int q = 0; q < P - 1; ++q) (void)issue(q);
-    int cs = 0; // step buffer consumed next
-    const unsigned xoff = g * ROWB + i * 16;
-    for (int64_t it = kq * kWaves + wave; it < total; it += wstride) {
-        const int64_t br = r_lo + (it >> cshift);
-        const int K = (int)(row_ptr[br + 1] - row_ptr[br]) * b;
-        const int nsteps = (K + 3) >> 2;
-        A4 acc[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = A4{0, 0, 0, 0};
-        for (int st = 0; st < nsteps; ++st) {
-            // refill the buffer the previous step read, then wait for this one
-            const bool more = issue(cs == 0 ? P - 1 : cs - 1);
-            const unsigned sb = wbuf_addr + cs * STB;
-            const unsigned xa = sb + xoff;
-            const unsigned aa = sb + XB + (g * b + i) * (int)sizeof(S);
-            F4 o[N16];
-            S a;
-            if (more) DmaRead<VM * (P - 1), N16, (int)sizeof(S)>::run(xa, aa, o, a);
-            else DmaRead<0, N16, (int)sizeof(S)>::run(xa, aa, o, a);
-            if (!(i < b && 4 * st + g < K)) a = S(0);
-            S bv[NT];
-            __builtin_memcpy(bv, o, sizeof(bv));
-#pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
-            cs = cs + 1 == P ? 0 : cs + 1;
-        }
-        mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
-    }
     mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
 }
 
@@ -909,57 +641,6 @@ hipError_t go_mfma_pd(const BsrDev &A, const void *X, void *Y, int W, int R, con
     return hipGetLastError();
 }
 
-template <typename S, int NT, int MODE, bool CV, int P>
-hipError_t go_dma_mode(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
-                       int grid, hipStream_t st) {
-    constexpr unsigned lds = kWaves * P * dma_step_bytes<S, NT>();
-    static bool attr_set = false; // one attribute call per kernel instance
-    if (!attr_set) {
-        hipError_t he = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&k_spmm_dma<S, NT, MODE, CV, P>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (he != hipSuccess) return he;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL((k_spmm_dma<S, NT, MODE, CV, P>), dim3(grid), dim3(64 * kWaves), lds, st,
-                       A.nbr, (int)A.b, A.row_ptr, A.xrow, static_cast<const S *>(A.vals),
-                       static_cast<const S *>(X), static_cast<S *>(Y), W, R, g_xcd_map, e);
-    return hipGetLastError();
-}
-
-template <typename S, int NT, bool CV, int P>
-hipError_t go_dma_p(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
-                    int grid, hipStream_t st) {
-    switch (e.mode) {
-    case EPI_PLAIN: return go_dma_mode<S, NT, EPI_PLAIN, CV, P>(A, X, Y, W, R, e, grid, st);
-    case EPI_FIRST: return go_dma_mode<S, NT, EPI_FIRST, CV, P>(A, X, Y, W, R, e, grid, st);
-    case EPI_STEP: return go_dma_mode<S, NT, EPI_STEP, CV, P>(A, X, Y, W, R, e, grid, st);
-    case EPI_PLAINSTEP: return go_dma_mode<S, NT, EPI_PLAINSTEP, CV, P>(A, X, Y, W, R, e, grid, st);
-    default: return go_dma_mode<S, NT, EPI_ADJ, CV, P>(A, X, Y, W, R, e, grid, st);
-    }
-}
-
-template <typename S, int NT, bool CV>
-hipError_t go_dma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e, int grid,
-                  hipStream_t st) {
-    if (g_dma_depth == 4) return go_dma_p<S, NT, CV, 4>(A, X, Y, W, R, e, grid, st);
-    return go_dma_p<S, NT, CV, 2>(A, X, Y, W, R, e, grid, st);
-}
-
-// NT * sizeof(S) >= 16: FP64 with NT >= 2, FP32 with NT >= 4
-template <typename S, bool CV>
-hipError_t dma_by_nt(int nt, const BsrDev &A, const void *X, void *Y, int W, int R,
-                     const EpiArgs &e, int grid, hipStream_t st) {
-    switch (nt) {
-    case 2:
-        if constexpr (sizeof(S) == 8) return go_dma<S, 2, CV>(A, X, Y, W, R, e, grid, st);
-        return hipErrorInvalidValue;
-    case 4: return go_dma<S, 4, CV>(A, X, Y, W, R, e, grid, st);
-    case 8: return go_dma<S, 8, CV>(A, X, Y, W, R, e, grid, st);
-    default: return go_dma<S, 16, CV>(A, X, Y, W, R, e, grid, st);
-    }
-}
-
 // register prefetch depth of k_spmm_mfma (g_mfma_pd64, g_mfma_pd32)
 template <typename S, int NT, bool CV>
 hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
@@ -982,262 +663,6 @@ hipError_t by_nt(int nt, const BsrDev &A, const void *X, void *Y, int W, int R, 
     case 4: return go_mfma<S, 4, CV>(A, X, Y, W, R, e, grid, st);
     case 8: return go_mfma<S, 8, CV>(A, X, Y, W, R, e, grid, st);
     default: return go_mfma<S, 16, CV>(A, X, Y, W, R, e, grid, st);
-    }
-}
-
-// ---------------------------------------------- union-staged MFMA SpMM
-// The 8 block rows of a workgroup (one wave each) gather, between them, far
-// fewer distinct X block panels than 8 x 17: consecutive diamond atoms share
-// neighbours, so a group of 8 needs ~56 distinct panels instead of 136. The
-// group's union list (sorted block columns, host-built once per matrix) is
-// streamed through LDS C panels at a time; each wave runs its own blocks that
-// fall in the staged chunk (K concatenated within the chunk) with its B
-// fragments read from LDS. Two workgroups per CU overlap one group's staging
-// with the other's MFMAs. Every wave owns all W columns (NT = W / 16).
-// LDS rows are swizzled in 16-byte units so that the 16 lanes reading one K
-// row (64 contiguous bytes each) hit distinct banks.
-constexpr int kGrpWaves = 8;
-
-template <typename S, int NT>
-__device__ __forceinline__ int grp_swz(int u) { // logical -> physical 16-byte unit
-    constexpr int NQ = NT * (int)sizeof(S) / 16;
-    if constexpr (NQ <= 1) {
-        return u;
-    } else {
-        const int i = u / NQ, q = u - (u / NQ) * NQ;
-        return i * NQ + (q ^ ((i * NQ / 16) & (NQ - 1)));
-    }
-}
-
-template <typename S, int NT, int MODE, bool CV>
-__global__ __launch_bounds__(512) void k_spmm_grp(int64_t nbr, int b,
-                                                  const int64_t *__restrict__ row_ptr,
-                                                  const int32_t *__restrict__ col_ind,
-                                                  const S *__restrict__ vals,
-                                                  const int64_t *__restrict__ grp_ptr,
-                                                  const int32_t *__restrict__ grp_cols,
-                                                  const int32_t *__restrict__ pos,
-                                                  const S *__restrict__ X, S *__restrict__ Y,
-                                                  int R, int C, int xcd_map, EpiArgs e) {
-    using A4 = typename Acc4<S>::T;
-    constexpr int NDOT = MODE == EPI_FIRST ? 3 : MODE == EPI_STEP ? 2 : MODE == EPI_PLAINSTEP ? 1 : 0;
-    constexpr int W = 16 * NT;                 // real columns, all in one wave
-    constexpr int UW = W * (int)sizeof(S) / 16; // 16-byte units per staged row
-    extern __shared__ float4 lds_units[];
-    const S *__restrict__ XS = static_cast<const S *>(e.xs);
-    const S *__restrict__ XP = static_cast<const S *>(e.xp);
-    const S *__restrict__ PSI = static_cast<const S *>(e.psi);
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = lane & 15;
-    const int g = lane >> 4;
-    const int64_t ngroups = (nbr + kGrpWaves - 1) / kGrpWaves;
-    // XCD x walks the x-th contiguous eighth of the groups
-    int64_t g_lo = 0, g_hi = ngroups, kq = blockIdx.x, nk = gridDim.x;
-    if (xcd_map) {
-        const int64_t per = (ngroups + 7) / 8;
-        const int x = blockIdx.x & 7;
-        kq = blockIdx.x >> 3;
-        nk = gridDim.x >> 3;
-        g_lo = x * per;
-        g_hi = min(ngroups, (int64_t)(x + 1) * per);
-    }
-
-    double dacc[NDOT > 0 ? NDOT : 1][NT];
-#pragma unroll
-    for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
-
-    for (int64_t grp = g_lo + kq; grp < g_hi; grp += nk) {
-        const int64_t br = grp * kGrpWaves + wave;
-        const bool has_row = br < nbr;
-        const int64_t u0 = grp_ptr[grp];
-        const int nu = (int)(grp_ptr[grp + 1] - u0);
-        int64_t sp = has_row ? row_ptr[br] : 0;
-        const int64_t s1 = has_row ? row_ptr[br + 1] : 0;
-
-        A4 acc[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = A4{0, 0, 0, 0};
-
-        for (int cu = 0; cu < nu; cu += C) {
-            const int nc = min(C, nu - cu);
-            // stage panels cu .. cu + nc - 1: b rows of W elements each
-            const int units = nc * b * UW;
-            for (int t = threadIdx.x; t < units; t += blockDim.x) {
-                const int rowl = t / UW, uu = t - rowl * UW;
-                const int j = rowl / b, kk = rowl - j * b;
-                const int64_t xr = (int64_t)grp_cols[u0 + cu + j] * b + kk;
-                const float4 v = reinterpret_cast<const float4 *>(X + xr * W)[uu];
-                lds_units[rowl * UW + grp_swz<S, NT>(uu)] = v;
-            }
-            __syncthreads();
-            if (has_row) {
-                int64_t se = sp;
-                while (se < s1 && pos[se] < cu + nc) ++se;
-                const int nk_ = (int)(se - sp) * b;
-                const int nsteps = (nk_ + 3) >> 2;
-                int blk = 0, kk = g;
-                while (kk >= b) { kk -= b; ++blk; }
-                for (int st = 0; st < nsteps; ++st) {
-                    S a = S(0);
-                    S bv[NT];
-                    if (4 * st + g < nk_) {
-                        const int64_t sb = sp + blk;
-                        if (i < b) a = __builtin_nontemporal_load(vals + sb * (int64_t)(b * b) + kk * b + i);
-                        const int rowl = (pos[sb] - cu) * b + kk;
-                        constexpr int NQ = NT * (int)sizeof(S) / 16;
-                        const float4 *src = lds_units + rowl * UW;
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            const float4 w = src[grp_swz<S, NT>(i * NQ + q)];
-                            __builtin_memcpy(reinterpret_cast<char *>(bv) + 16 * q, &w, 16);
-                        }
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NT; ++t) bv[t] = S(0);
-                    }
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) acc[t] = mfma16(a, bv[t], acc[t]);
-                    kk += 4;
-                    while (kk >= b) { kk -= b; ++blk; }
-                }
-                sp = se;
-            }
-            __syncthreads();
-        }
-
-        if (!has_row) continue;
-#pragma unroll 1
-        for (int v = 0; v < 4; ++v) {
-            const int r = sizeof(S) == 8 ? 4 * v + g : 4 * g + v;
-            if (r >= b) continue;
-            const int64_t base = (br * b + r) * (int64_t)W + i * NT;
-            S y[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-                y[t] = v == 0 ? acc[t][0] : v == 1 ? acc[t][1] : v == 2 ? acc[t][2] : acc[t][3];
-            if constexpr (MODE != EPI_PLAIN) {
-                S xs[NT];
-                load_row<S, NT>(XS + base, xs);
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
-                if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
-                    S xp[NT];
-                    load_row<S, NT>(XP + base, xp);
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
-                }
-                if constexpr (MODE == EPI_ADJ) {
-                    const S *qs[3] = {static_cast<const S *>(e.qm), static_cast<const S *>(e.qz),
-                                      static_cast<const S *>(e.qp)};
-                    const double cs[3] = {e.cm, e.cz, e.cp};
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        S qv[NT];
-                        load_row<S, NT>(qs[q] + base, qv);
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
-                    }
-                }
-                auto dot = [&](int d, const S (&a)[NT], const S (&bb)[NT], bool self) {
-                    if constexpr (CV) {
-#pragma unroll
-                        for (int t = 0; t < NT; t += 2) {
-                            const double ar = a[t], ai = a[t + 1], br_ = bb[t], bi = bb[t + 1];
-                            dacc[d][t] = __builtin_fma(ar, br_, __builtin_fma(ai, bi, dacc[d][t]));
-                            if (!self)
-                                dacc[d][t + 1] = __builtin_fma(ar, bi, __builtin_fma(-ai, br_, dacc[d][t + 1]));
-                        }
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            dacc[d][t] = __builtin_fma((double)a[t], (double)bb[t], dacc[d][t]);
-                    }
-                };
-                if constexpr (MODE == EPI_FIRST) {
-                    dot(0, xs, xs, true);
-                    dot(1, xs, y, false);
-                    dot(2, y, y, true);
-                } else if constexpr (MODE == EPI_STEP) {
-                    dot(0, y, y, true);
-                    dot(1, y, xs, false);
-                } else if constexpr (MODE == EPI_PLAINSTEP) {
-                    S ps[NT];
-                    load_row<S, NT>(PSI + base, ps);
-                    dot(0, ps, y, false);
-                }
-            }
-            store_row<S, NT>(Y + base, y);
-        }
-    }
-
-    if constexpr (NDOT > 0) {
-#pragma unroll
-        for (int d = 0; d < NDOT; ++d)
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                dacc[d][t] += __shfl_xor(dacc[d][t], 16);
-                dacc[d][t] += __shfl_xor(dacc[d][t], 32);
-            }
-        constexpr int UPC = CV ? W / 2 : W;
-        double2 *sm = reinterpret_cast<double2 *>(lds_units); // staging area is free now
-        for (int d = 0; d < NDOT; ++d) {
-            if (g == 0) {
-#pragma unroll
-                for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
-                    const int uu = CV ? (i * NT + t) / 2 : i * NT + t;
-                    sm[wave * UPC + uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
-                }
-            }
-            __syncthreads();
-            for (int u = threadIdx.x; u < R; u += blockDim.x) {
-                double2 a2 = double2{0.0, 0.0};
-                for (int w = 0; w < kGrpWaves; ++w) {
-                    a2.x += sm[w * UPC + u].x;
-                    a2.y += sm[w * UPC + u].y;
-                }
-                e.partials[((int64_t)blockIdx.x * NDOT + d) * R + u] = a2;
-            }
-            __syncthreads();
-        }
-    }
-}
-
-template <typename S, int NT, bool CV>
-hipError_t go_grp(const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, int grid,
-                  int C, size_t lds, hipStream_t st) {
-    const dim3 gd(grid), bd(64 * kGrpWaves);
-    const S *vals = static_cast<const S *>(A.vals);
-    const S *x = static_cast<const S *>(X);
-    S *y = static_cast<S *>(Y);
-    const int xcd = g_xcd_map;
-#define RG_GRP_LAUNCH(MODE)                                                                        \
-    hipLaunchKernelGGL((k_spmm_grp<S, NT, MODE, CV>), gd, bd, lds, st, A.nbr, (int)A.b, A.row_ptr,  \
-                       A.col_ind, vals, A.grp_ptr, A.grp_cols, A.pos, x, y, R, C, xcd, e)
-    switch (e.mode) {
-    case EPI_PLAIN: RG_GRP_LAUNCH(EPI_PLAIN); break;
-    case EPI_FIRST: RG_GRP_LAUNCH(EPI_FIRST); break;
-    case EPI_STEP: RG_GRP_LAUNCH(EPI_STEP); break;
-    case EPI_PLAINSTEP: RG_GRP_LAUNCH(EPI_PLAINSTEP); break;
-    default: RG_GRP_LAUNCH(EPI_ADJ); break;
-    }
-#undef RG_GRP_LAUNCH
-    return hipGetLastError();
-}
-
-template <typename S, bool CV>
-hipError_t grp_by_nt(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
-                     int grid, int C, size_t lds, hipStream_t st) {
-    switch (W / 16) {
-    case 2: return go_grp<S, 2, CV>(A, X, Y, R, e, grid, C, lds, st);
-    case 4: return go_grp<S, 4, CV>(A, X, Y, R, e, grid, C, lds, st);
-    case 8: return go_grp<S, 8, CV>(A, X, Y, R, e, grid, C, lds, st);
-    default: return go_grp<S, 16, CV>(A, X, Y, R, e, grid, C, lds, st);
     }
 }
 
@@ -1394,29 +819,6 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
     if (R < 1 || R > 256 || A.b < 1 || A.b > 16) return hipErrorInvalidValue;
     if (A.is_complex && !vm.complex_vec) return hipErrorInvalidValue;
     const int W = R * (vm.complex_vec ? 2 : 1);
-    const size_t elem = vm.fp32 ? 4 : 8;
-    // each lane owns W/16 columns; they must fill whole 16-byte LDS units
-    if (g_union_stage && !A.is_complex && A.b >= 8 && A.grp_ptr &&
-        (W == 32 || W == 64 || W == 128 || W == 256) && W * elem >= 256) {
-        const size_t panel = (size_t)A.b * W * elem;
-        int C = (int)(kGrpLdsBudget / panel);
-        if (C > 16) C = 16;
-        if (C >= 1) {
-            const int64_t ngroups = (A.nbr + kGrpWaves - 1) / kGrpWaves;
-            int64_t g = g_xcd_map ? ((ngroups + 7) / 8) * 8 : ngroups;
-            if (g > kMaxGrid) g = kMaxGrid; // grid-stride over groups (a multiple of 8)
-            {
-                const int grid = (int)g;
-                if (grid_out) *grid_out = grid;
-                const size_t lds = std::max((size_t)C * panel, (size_t)kGrpWaves * (W / (vm.complex_vec ? 2 : 1)) * 16);
-                if (vm.fp32)
-                    return vm.complex_vec ? grp_by_nt<float, true>(A, X, Y, W, R, e, grid, C, lds, stream)
-                                          : grp_by_nt<float, false>(A, X, Y, W, R, e, grid, C, lds, stream);
-                return vm.complex_vec ? grp_by_nt<double, true>(A, X, Y, W, R, e, grid, C, lds, stream)
-                                      : grp_by_nt<double, false>(A, X, Y, W, R, e, grid, C, lds, stream);
-            }
-        }
-    }
     int nt = mfma_nt(W);
     while (nt > 2 && nt > g_mfma_ntcap) nt /= 2;
     const int nchunks = nt > 0 ? W / (16 * nt) : 0;
@@ -1426,14 +828,6 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
         g = ((g + 7) / 8) * 8; // whole XCD rounds (kMaxGrid is a multiple of 8)
         const int grid = (int)g;
         if (grid_out) *grid_out = grid;
-        if (g_dma_depth > 0 && A.xrow && nt * (vm.fp32 ? 4 : 8) >= 16 && vm.fp32 &&
-            e.mode != EPI_PLAIN) {
-            if (vm.fp32)
-                return vm.complex_vec ? dma_by_nt<float, true>(nt, A, X, Y, W, R, e, grid, stream)
-                                      : dma_by_nt<float, false>(nt, A, X, Y, W, R, e, grid, stream);
-            return vm.complex_vec ? dma_by_nt<double, true>(nt, A, X, Y, W, R, e, grid, stream)
-                                  : dma_by_nt<double, false>(nt, A, X, Y, W, R, e, grid, stream);
-        }
         if (vm.fp32) {
             return vm.complex_vec ? by_nt<float, true>(nt, A, X, Y, W, R, e, grid, stream)
                                   : by_nt<float, false>(nt, A, X, Y, W, R, e, grid, stream);
@@ -1457,11 +851,8 @@ int spmm_max_grid() { return kMaxGrid; }
 
 void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
-void set_union_stage(int on) { g_union_stage = on ? 1 : 0; }
 void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
 void set_mfma_pd64(int p) { g_mfma_pd64 = p == 2 ? 2 : 3; }
-void set_dma_depth(int p) { g_dma_depth = (p == 0 || p == 4) ? p : 2; }
-int union_group_rows() { return kGrpWaves; }
 
 hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
                                   hipStream_t stream) {
