@@ -195,6 +195,47 @@ RG_API int rg_bsr_info(rg_context *ctx, int slot, int64_t *n_block_rows, int *bl
 RG_API int rg_download_bsr(rg_context *ctx, int slot, int64_t *row_ptr, int32_t *col_ind,
                            double *values);
 
+/* ---- Slater-Koster assembly on the device (SkTableConstruction::build_images,
+ *      rsatb/model/sktb/sktb_construction.hpp:327-425, over from_graph arrays,
+ *      rsatb/pybind_rsatb.cpp:562-593) ---- */
+
+/* One matrix (H or S) of a resolved SK table (sk_table.hpp:104-161), as
+ * plain arrays. Every type has norb orbitals (the device BSR is uniform).
+ * Orbital o of type t: angular momentum orb_l[t norb + o] (0, 1, 2) and
+ * magnetic index orb_m[t norb + o] (DeePTB order: py pz px; dxy dyz dz2 dzx
+ * dx2-y2); onsite[t norb + o] its onsite diagonal. pair_cutoff[rt T + ct]:
+ * the largest cutoff of this matrix's channels of the ordered type pair, 0
+ * when it has none. Channel c: types, shells (row_l, col_l), bond (0 sigma,
+ * 1 pi, 2 delta, at most min(row_l, col_l)), radial kind (0 natural cubic
+ * spline through rad_x/rad_y with second derivatives rad_y2; 1 exp_poly with
+ * coefficients rad_y, decay lambda and cosine taper from r_on), its data at
+ * rad_*[ch_off .. ch_off + ch_n), and cutoff ch_r_cut. A shell pair l < l'
+ * appears as both (l, l') and (l', l) (sk_table.hpp:404-407). */
+typedef struct rg_sk_table {
+    int32_t n_types, norb, n_channels;
+    const int32_t *orb_l, *orb_m;
+    const double *onsite;
+    const double *pair_cutoff;
+    const int32_t *ch_row_type, *ch_col_type, *ch_row_l, *ch_col_l, *ch_bond, *ch_kind, *ch_off, *ch_n;
+    const double *ch_r_cut, *ch_lambda, *ch_r_on;
+    const double *rad_x, *rad_y, *rad_y2;
+    int64_t n_rad;
+} rg_sk_table;
+
+/* Builds H(k) or S(k) of the table into `slot` on the device: the onsite
+ * diagonal block of every atom, and for each edge (edge_index[2e] ->
+ * edge_index[2e + 1], bond vector pos[j] + shift . cell - pos[i], cell rows
+ * the lattice vectors) the SK block of the table pair of its types, else the
+ * transposed block of the reversed pair (sktb_construction.hpp:476-500);
+ * edges whose block is empty or has no entry above 1e-14 are not stored
+ * (:385). Blocks of the same (i, j) are summed with phases exp(2 pi i k.shift)
+ * (kpoint NULL or zero: real values). The slot then serves the moment loop
+ * like an uploaded matrix; *nnzb_out receives the stored block count. */
+RG_API int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *table, int64_t n_atoms,
+                          const int32_t *atom_type, const double *pos, const double *cell,
+                          int64_t n_edges, const int32_t *edge_index, const int32_t *edge_shift,
+                          const double *kpoint, int64_t *nnzb_out);
+
 /* ---- graph-local S^{-1/2} (graph_inverse_sqrt seed, rsatb/calc/isqrtm.hpp:137-232;
  *      ISQRTMPreconditioner ctor, rsatb/calc/preconditioner.hpp:184-192) ---- */
 

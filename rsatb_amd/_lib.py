@@ -37,6 +37,21 @@ _vp, _i, _i64, _d, _u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64
 _pd = C.POINTER(C.c_double)
 _pi64 = C.POINTER(C.c_int64)
 _pi32 = C.POINTER(C.c_int32)
+
+
+class SkTableC(C.Structure):
+    """struct rg_sk_table (include/rsatb_gpu.h)."""
+
+    _fields_ = [
+        ("n_types", C.c_int32), ("norb", C.c_int32), ("n_channels", C.c_int32),
+        ("orb_l", _pi32), ("orb_m", _pi32), ("onsite", _pd), ("pair_cutoff", _pd),
+        ("ch_row_type", _pi32), ("ch_col_type", _pi32), ("ch_row_l", _pi32), ("ch_col_l", _pi32),
+        ("ch_bond", _pi32), ("ch_kind", _pi32), ("ch_off", _pi32), ("ch_n", _pi32),
+        ("ch_r_cut", _pd), ("ch_lambda", _pd), ("ch_r_on", _pd),
+        ("rad_x", _pd), ("rad_y", _pd), ("rad_y2", _pd), ("n_rad", C.c_int64),
+    ]
+
+
 SIGNATURES = {
     "rg_version": (_i, []),
     "rg_create": (_i, [_i, C.POINTER(_vp)]),
@@ -55,6 +70,8 @@ SIGNATURES = {
     "rg_kpm_density": (_i, [_pd, _i, _pd, _i64, _d, _i, _pd]),
     "rg_bsr_info": (_i, [_vp, _i, _pi64, C.POINTER(_i), _pi64, C.POINTER(_i)]),
     "rg_download_bsr": (_i, [_vp, _i, _pi64, _pi32, _pd]),
+    "rg_sk_assemble": (_i, [_vp, _i, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32, _pd,
+                            _pi64]),
     "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_set_checkpointing": (_i, [_vp, _i]),
@@ -194,6 +211,45 @@ class Context:
                                        _dptr(vals.view(np.float64)))
         _check(self._h, rc, "rg_download_bsr")
         return BSR(nbr, b, rp, ci[:nnzb], vals[:nnzb])
+
+    def sk_assemble(self, slot: int, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift,
+                    kpoint=None) -> int:
+        """Builds H(k) or S(k) of an SK table on the device into `slot`
+        (table_arrays: rsatb_amd.sktb.SkTable.arrays(which); edge_index as
+        flattened (src, dst) pairs). Returns the stored block count."""
+        a = table_arrays
+        keep = {}
+
+        def i32(name):
+            keep[name] = np.ascontiguousarray(a[name], dtype=np.int32)
+            return keep[name].ctypes.data_as(_pi32)
+
+        def f64(name):
+            keep[name] = np.ascontiguousarray(a[name], dtype=np.float64)
+            return keep[name].ctypes.data_as(_pd)
+
+        t = SkTableC(int(a["n_types"]), int(a["norb"]), int(a["n_channels"]), i32("orb_l"), i32("orb_m"),
+                     f64("onsite"), f64("pair_cutoff"), i32("ch_row_type"), i32("ch_col_type"), i32("ch_row_l"),
+                     i32("ch_col_l"), i32("ch_bond"), i32("ch_kind"), i32("ch_off"), i32("ch_n"),
+                     f64("ch_r_cut"), f64("ch_lam"), f64("ch_r_on"), f64("rad_x"), f64("rad_y"), f64("rad_y2"),
+                     int(np.asarray(a["rad_y"]).size))
+        types = np.ascontiguousarray(atom_type, dtype=np.int32).reshape(-1)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+        ei = np.ascontiguousarray(edge_index, dtype=np.int32).reshape(-1)
+        sh = np.ascontiguousarray(edge_shift, dtype=np.int32).reshape(-1)
+        if ei.size % 2 or sh.size != 3 * (ei.size // 2):
+            raise ValueError("edge_index needs (src, dst) pairs and edge_shift three integers per edge")
+        if pos.shape[0] != types.size:
+            raise ValueError("pos and atom_type describe different atom counts")
+        kp = None if kpoint is None else np.ascontiguousarray(kpoint, dtype=np.float64).reshape(3)
+        nnzb = C.c_int64()
+        rc = self._lib.rg_sk_assemble(self._h, int(slot), C.byref(t), int(types.size), types.ctypes.data_as(_pi32),
+                                      _dptr(pos), _dptr(cell), int(ei.size // 2), ei.ctypes.data_as(_pi32),
+                                      sh.ctypes.data_as(_pi32), None if kp is None else _dptr(kp),
+                                      C.byref(nnzb))
+        _check(self._h, rc, "rg_sk_assemble")
+        return nnzb.value
 
     def isqrt_patches(self, tol: float = 1e-12, maxit: int = 60) -> tuple[int, float]:
         """Z, Z^H from the overlap in SLOT_S (graph_inverse_sqrt seed)."""

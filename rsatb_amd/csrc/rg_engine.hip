@@ -945,6 +945,219 @@ int rg_download_bsr(rg_context *ctx, int slot, int64_t *row_ptr, int32_t *col_in
     return RG_OK;
 }
 
+int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, const int32_t *atom_type,
+                   const double *pos, const double *cell, int64_t n_edges, const int32_t *edge_index,
+                   const int32_t *edge_shift, const double *kpoint, int64_t *nnzb_out) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (slot < 0 || slot > 3) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
+    if (!t || !atom_type || !pos || !cell) return fail(ctx, RG_ERR_INVALID, "null array");
+    if (n_edges < 0 || (n_edges > 0 && (!edge_index || !edge_shift)))
+        return fail(ctx, RG_ERR_INVALID, "edge arrays missing");
+    const int T = t->n_types, b = t->norb, NC = t->n_channels;
+    if (T < 1 || b < 1 || b > 16 || NC < 0) return fail(ctx, RG_ERR_INVALID, "sk table: n_types >= 1, 1 <= norb <= 16");
+    if (n_atoms < 1 || n_atoms > INT32_MAX) return fail(ctx, RG_ERR_INVALID, "n_atoms out of range");
+    if (n_edges > INT32_MAX) return fail(ctx, RG_ERR_UNSUPPORTED, "at most 2^31 - 1 edges");
+    if (!t->orb_l || !t->orb_m || !t->onsite || !t->pair_cutoff)
+        return fail(ctx, RG_ERR_INVALID, "sk table: null array");
+    if (NC > 0 && (!t->ch_row_type || !t->ch_col_type || !t->ch_row_l || !t->ch_col_l || !t->ch_bond ||
+                   !t->ch_kind || !t->ch_off || !t->ch_n || !t->ch_r_cut || !t->ch_lambda || !t->ch_r_on ||
+                   !t->rad_x || !t->rad_y || !t->rad_y2 || t->n_rad < 1))
+        return fail(ctx, RG_ERR_INVALID, "sk table: null channel array");
+    for (int q = 0; q < T * b; ++q)
+        if (t->orb_l[q] < 0 || t->orb_l[q] > 2 || t->orb_m[q] < 0 || t->orb_m[q] > 2 * t->orb_l[q])
+            return fail(ctx, RG_ERR_INVALID, "sk table: orbital l must be 0..2 and m 0..2l");
+    for (int c = 0; c < NC; ++c) {
+        const int rl = t->ch_row_l[c], cl = t->ch_col_l[c];
+        if (t->ch_row_type[c] < 0 || t->ch_row_type[c] >= T || t->ch_col_type[c] < 0 || t->ch_col_type[c] >= T)
+            return fail(ctx, RG_ERR_INVALID, "sk table: channel type out of range");
+        if (rl < 0 || rl > 2 || cl < 0 || cl > 2 || t->ch_bond[c] < 0 || t->ch_bond[c] > std::min(rl, cl))
+            return fail(ctx, RG_ERR_INVALID, "sk table: channel shells or bond out of range");
+        if (t->ch_kind[c] != 0 && t->ch_kind[c] != 1)
+            return fail(ctx, RG_ERR_INVALID, "sk table: radial kind must be 0 (spline) or 1 (exp_poly)");
+        if (t->ch_n[c] < 1 || t->ch_off[c] < 0 || (int64_t)t->ch_off[c] + t->ch_n[c] > t->n_rad)
+            return fail(ctx, RG_ERR_INVALID, "sk table: radial data out of range");
+        if (!(t->ch_r_cut[c] > 0.0)) return fail(ctx, RG_ERR_INVALID, "sk table: radial cutoff must be positive");
+    }
+    for (int64_t a = 0; a < n_atoms; ++a)
+        if (atom_type[a] < 0 || atom_type[a] >= T) return fail(ctx, RG_ERR_INVALID, "atom type out of range");
+    for (int64_t e = 0; e < 2 * n_edges; ++e)
+        if (edge_index[e] < 0 || edge_index[e] >= n_atoms)
+            return fail(ctx, RG_ERR_INVALID, "edge atom index out of range");
+    const bool cplx = kpoint && (kpoint[0] != 0.0 || kpoint[1] != 0.0 || kpoint[2] != 0.0);
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    RG_HIP(ctx, sk_upload_constants(), "SK constants");
+
+    // channels grouped by ordered type pair (stable), with per-pair ranges
+    std::vector<int> order(NC);
+    for (int c = 0; c < NC; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return t->ch_row_type[x] * T + t->ch_col_type[x] < t->ch_row_type[y] * T + t->ch_col_type[y];
+    });
+    std::vector<int32_t> ints; // orb_l, orb_m, pair_ch, row_l, col_l, bond, kind, off, n
+    std::vector<double> dbls;  // onsite, pair_cutoff, r_cut, lambda, r_on, rad_x, rad_y, rad_y2
+    ints.insert(ints.end(), t->orb_l, t->orb_l + T * b);
+    ints.insert(ints.end(), t->orb_m, t->orb_m + T * b);
+    std::vector<int32_t> pair_ch(2 * T * T, 0);
+    for (int q = 0; q < NC;) {
+        const int key = t->ch_row_type[order[q]] * T + t->ch_col_type[order[q]];
+        int q1 = q;
+        while (q1 < NC && t->ch_row_type[order[q1]] * T + t->ch_col_type[order[q1]] == key) ++q1;
+        pair_ch[2 * key] = q;
+        pair_ch[2 * key + 1] = q1;
+        q = q1;
+    }
+    ints.insert(ints.end(), pair_ch.begin(), pair_ch.end());
+    for (const int32_t *arr : {t->ch_row_l, t->ch_col_l, t->ch_bond, t->ch_kind, t->ch_off, t->ch_n})
+        for (int c = 0; c < NC; ++c) ints.push_back(arr[order[c]]);
+    dbls.insert(dbls.end(), t->onsite, t->onsite + T * b);
+    dbls.insert(dbls.end(), t->pair_cutoff, t->pair_cutoff + T * T);
+    for (const double *arr : {t->ch_r_cut, t->ch_lambda, t->ch_r_on})
+        for (int c = 0; c < NC; ++c) dbls.push_back(arr[order[c]]);
+    const int64_t nrad = NC > 0 ? t->n_rad : 0;
+    for (const double *arr : {t->rad_x, t->rad_y, t->rad_y2})
+        if (nrad) dbls.insert(dbls.end(), arr, arr + nrad);
+    ints.push_back(0); // keeps every array non-empty
+    dbls.push_back(0.0);
+
+    DevBuf dints, ddbl, dtype, dpos, dei, dsh, deblk, dflags;
+    auto cleanup = [&]() {
+        for (DevBuf *x : {&dints, &ddbl, &dtype, &dpos, &dei, &dsh, &deblk, &dflags}) x->release();
+    };
+    auto up = [&](DevBuf &d, const void *h, size_t bytes) -> hipError_t {
+        hipError_t e = d.ensure(bytes ? bytes : 16);
+        if (e == hipSuccess && bytes) e = hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
+        return e;
+    };
+    hipError_t he = up(dints, ints.data(), ints.size() * 4);
+    if (he == hipSuccess) he = up(ddbl, dbls.data(), dbls.size() * 8);
+    if (he == hipSuccess) he = up(dtype, atom_type, (size_t)n_atoms * 4);
+    if (he == hipSuccess) he = up(dpos, pos, (size_t)n_atoms * 24);
+    if (he == hipSuccess) he = up(dei, edge_index, (size_t)n_edges * 8);
+    if (he == hipSuccess) he = up(dsh, edge_shift, (size_t)n_edges * 12);
+    if (he == hipSuccess) he = deblk.ensure((size_t)std::max<int64_t>(n_edges, 1) * b * b * 8);
+    if (he == hipSuccess) he = dflags.ensure((size_t)std::max<int64_t>(n_edges, 1));
+    if (he != hipSuccess) {
+        cleanup();
+        return hipfail(ctx, he, "sk assemble: upload");
+    }
+    SkDev sd;
+    sd.n_types = T;
+    sd.norb = b;
+    sd.n_channels = NC;
+    {
+        const int32_t *ip = static_cast<const int32_t *>(dints.p);
+        sd.orb_l = ip;
+        sd.orb_m = ip + T * b;
+        sd.pair_ch = ip + 2 * T * b;
+        const int32_t *cp = ip + 2 * T * b + 2 * T * T;
+        sd.ch_row_l = cp;
+        sd.ch_col_l = cp + NC;
+        sd.ch_bond = cp + 2 * NC;
+        sd.ch_kind = cp + 3 * NC;
+        sd.ch_off = cp + 4 * NC;
+        sd.ch_n = cp + 5 * NC;
+        const double *dp = static_cast<const double *>(ddbl.p);
+        sd.onsite = dp;
+        sd.pair_cutoff = dp + T * b;
+        const double *cd = dp + T * b + T * T;
+        sd.ch_r_cut = cd;
+        sd.ch_lambda = cd + NC;
+        sd.ch_r_on = cd + 2 * NC;
+        sd.rad_x = cd + 3 * NC;
+        sd.rad_y = sd.rad_x + nrad;
+        sd.rad_y2 = sd.rad_y + nrad;
+    }
+    SkCell sc;
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) sc.c[r][q] = cell[3 * r + q];
+    std::vector<uint8_t> flags((size_t)n_edges);
+    he = launch_sk_edge_blocks(sd, n_edges, static_cast<const int32_t *>(dtype.p),
+                               static_cast<const double *>(dpos.p), sc, static_cast<const int32_t *>(dei.p),
+                               static_cast<const int32_t *>(dsh.p), static_cast<double *>(deblk.p),
+                               static_cast<uint8_t *>(dflags.p), ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he == hipSuccess && n_edges)
+        he = hipMemcpy(flags.data(), dflags.p, (size_t)n_edges, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) {
+        cleanup();
+        return hipfail(ctx, he, "sk assemble: edge blocks");
+    }
+
+    // pattern: (row, col, source) sorted; onsite (-1) first within a block,
+    // then its edges in ascending order
+    struct Src {
+        int32_t row, col, src;
+    };
+    std::vector<Src> srcs;
+    srcs.reserve((size_t)n_atoms + (size_t)n_edges);
+    for (int64_t a = 0; a < n_atoms; ++a) srcs.push_back({(int32_t)a, (int32_t)a, -1});
+    for (int64_t e = 0; e < n_edges; ++e)
+        if (flags[e]) srcs.push_back({edge_index[2 * e], edge_index[2 * e + 1], (int32_t)e});
+    std::sort(srcs.begin(), srcs.end(), [](const Src &x, const Src &y) {
+        if (x.row != y.row) return x.row < y.row;
+        if (x.col != y.col) return x.col < y.col;
+        return x.src < y.src;
+    });
+    std::vector<int64_t> row_ptr((size_t)n_atoms + 1, 0), src_ptr(1, 0);
+    std::vector<int32_t> col_ind, src_id, row_of;
+    std::vector<double> phase;
+    for (size_t q = 0; q < srcs.size(); ++q) {
+        const bool first = q == 0 || srcs[q].row != srcs[q - 1].row || srcs[q].col != srcs[q - 1].col;
+        if (first) {
+            if (q) src_ptr.push_back((int64_t)q);
+            col_ind.push_back(srcs[q].col);
+            row_of.push_back(srcs[q].row);
+            ++row_ptr[srcs[q].row + 1];
+        }
+        src_id.push_back(srcs[q].src);
+        if (cplx) {
+            double arg = 0.0;
+            if (srcs[q].src >= 0)
+                for (int k = 0; k < 3; ++k) arg += kpoint[k] * edge_shift[3 * (int64_t)srcs[q].src + k];
+            phase.push_back(std::cos(2.0 * M_PI * arg));
+            phase.push_back(std::sin(2.0 * M_PI * arg));
+        }
+    }
+    src_ptr.push_back((int64_t)srcs.size());
+    for (int64_t a = 0; a < n_atoms; ++a) row_ptr[a + 1] += row_ptr[a];
+    const int64_t nnzb = (int64_t)col_ind.size();
+
+    Slot &s = ctx->slots[slot];
+    s.present = false;
+    s.vals32_valid = false;
+    DevBuf dsp, dsrc, dph, drow;
+    const size_t nv = (size_t)nnzb * b * b * (cplx ? 2 : 1);
+    he = up(dsp, src_ptr.data(), src_ptr.size() * 8);
+    if (he == hipSuccess) he = up(dsrc, src_id.data(), src_id.size() * 4);
+    if (he == hipSuccess) he = up(dph, phase.data(), phase.size() * 8);
+    if (he == hipSuccess) he = up(drow, row_of.data(), row_of.size() * 4);
+    if (he == hipSuccess) he = s.row_ptr.ensure((size_t)(n_atoms + 1) * 8);
+    if (he == hipSuccess) he = s.col_ind.ensure((size_t)nnzb * 4 + 4);
+    if (he == hipSuccess) he = s.vals64.ensure(nv * 8);
+    if (he == hipSuccess)
+        he = hipMemcpy(s.row_ptr.p, row_ptr.data(), (size_t)(n_atoms + 1) * 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(s.col_ind.p, col_ind.data(), (size_t)nnzb * 4, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+        he = launch_sk_gather(nnzb, b, static_cast<const int64_t *>(dsp.p), static_cast<const int32_t *>(dsrc.p),
+                              static_cast<const double2 *>(dph.p), static_cast<const int32_t *>(drow.p),
+                              static_cast<const int32_t *>(dtype.p), sd.onsite,
+                              static_cast<const double *>(deblk.p), static_cast<double *>(s.vals64.p), cplx,
+                              ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    for (DevBuf *x : {&dsp, &dsrc, &dph, &drow}) x->release();
+    cleanup();
+    if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: gather");
+    s.nbr = n_atoms;
+    s.b = b;
+    s.nnzb = nnzb;
+    s.is_complex = cplx;
+    int rcg = build_xrow(ctx, s, col_ind.data());
+    if (rcg) return rcg;
+    s.present = true;
+    if (nnzb_out) *nnzb_out = nnzb;
+    return RG_OK;
+}
+
 int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, double *resid_out) {
     if (!ctx) return RG_ERR_INVALID;
     Slot &S = ctx->slots[RG_SLOT_S];

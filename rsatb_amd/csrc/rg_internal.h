@@ -135,4 +135,36 @@ hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *ro
 // y = 0 over n doubles
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st);
 
+// ---- Slater-Koster assembly (rg_sk.hip) ----
+// Device copy of one matrix's SK table; channels grouped by (row type, col
+// type), pair_ch[2 (rt T + ct)] .. [+1] the range of a pair's channels.
+struct SkDev {
+    int n_types = 0, norb = 0, n_channels = 0;
+    const int32_t *orb_l = nullptr, *orb_m = nullptr; // [T * norb]
+    const double *onsite = nullptr;                   // [T * norb]
+    const double *pair_cutoff = nullptr;              // [T * T]
+    const int32_t *pair_ch = nullptr;                 // [T * T * 2]
+    const int32_t *ch_row_l = nullptr, *ch_col_l = nullptr, *ch_bond = nullptr, *ch_kind = nullptr;
+    const int32_t *ch_off = nullptr, *ch_n = nullptr;
+    const double *ch_r_cut = nullptr, *ch_lambda = nullptr, *ch_r_on = nullptr;
+    const double *rad_x = nullptr, *rad_y = nullptr, *rad_y2 = nullptr;
+};
+// lattice vectors as rows
+struct SkCell {
+    double c[3][3];
+};
+hipError_t sk_upload_constants();
+// every edge's row-major block into eblk[e b^2 ..], and flags[e] = any
+// |entry| > 1e-14
+hipError_t launch_sk_edge_blocks(const SkDev &t, int64_t n_edges, const int32_t *type, const double *pos,
+                                 const SkCell &cell, const int32_t *edge_index, const int32_t *edge_shift,
+                                 double *eblk, uint8_t *flags, hipStream_t st);
+// stored block s = sum over q in [src_ptr[s], src_ptr[s+1]) of source src[q]
+// (-1: the onsite diagonal of row_of[s]'s atom, else edge src[q]) times
+// phase[q] when cplx; written column-major (interleaved re, im when cplx)
+hipError_t launch_sk_gather(int64_t nnzb, int b, const int64_t *src_ptr, const int32_t *src,
+                            const double2 *phase, const int32_t *row_of, const int32_t *type,
+                            const double *onsite, const double *eblk, double *vals, bool cplx,
+                            hipStream_t st);
+
 } // namespace rg
