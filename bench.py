@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward leg")
     p.add_argument("--adj-cells", type=int, default=128, help="graphene rectangular cells per edge (adjoint leg)")
+    p.add_argument("--no-sk", action="store_true", help="skip the Slater-Koster assembly leg")
+    p.add_argument("--sk-cells", type=int, default=16, help="diamond Si cells per edge (SK assembly leg)")
+    p.add_argument("--sk-cpu-edges", type=int, default=20000, help="edge blocks the CPU oracle evaluates")
     return p.parse_args()
 
 
@@ -184,6 +187,69 @@ def adjoint_leg(cells: int, steps: int) -> dict:
             "bwd_fwd": round(t_b / t_f, 3), "grad_theta": [float(x) for x in grad]}
 
 
+def sk_silicon_params():
+    """(basis, H, S) of a synthetic sp3d5 silicon model, every SK channel
+    (s, p, d shells), cutoff 4.1 A (first and second neighbour shells of
+    diamond Si), half the channels natural splines and half exp_poly."""
+    import numpy as np
+
+    from rsatb_amd.sktb import Radial, SkParameters
+
+    h, s = SkParameters(), SkParameters()
+    h.onsite["Si"] = {"s": -4.2, "p": 1.7, "d": 6.5}
+    vals = [("ss", "sigma", -1.9), ("sp", "sigma", 2.1), ("pp", "sigma", 3.0), ("pp", "pi", -0.9),
+            ("sd", "sigma", -1.6), ("pd", "sigma", -1.3), ("pd", "pi", 1.9), ("dd", "sigma", -1.2),
+            ("dd", "pi", 2.2), ("dd", "delta", -1.7)]
+    r = np.linspace(2.0, 4.1, 10)
+    for i, (sp, bond, v) in enumerate(vals):
+        if i % 2:
+            h.set_pair_channel("Si", "Si", sp, bond, Radial.exp_poly([v, 0.1 * v, -0.05 * v], 4.1, 0.4, 3.4))
+        else:
+            h.set_pair_channel("Si", "Si", sp, bond, Radial.spline(r, v * np.exp(-(r - 2.35)), 4.1))
+        s.set_pair_channel("Si", "Si", sp, bond, Radial.spline(r, 0.03 * v * np.exp(-(r - 2.35)), 4.1))
+    return {"Si": ["s", "p", "d"]}, h, s
+
+
+def sk_leg(cells: int, reps: int, cpu_edges: int) -> dict:
+    """Slater-Koster assembly of H and S on the device (rg_sk_assemble) for
+    diamond Si, cells^3 conventional cells, 9 orbitals, 16 neighbours per
+    atom; the CPU oracle timed on the first cpu_edges edge blocks."""
+    import numpy as np
+
+    from oracle import sk_oracle as so
+    from rsatb_amd import _lib
+    from rsatb_amd.lattice import diamond_supercell, neighbor_edges
+    from rsatb_amd.sktb import SktbModel
+
+    st = diamond_supercell(cells, a=5.43)
+    model = SktbModel(*sk_silicon_params())
+    e = neighbor_edges(st, model.cutoff)
+    pairs = np.stack([e.src, e.dst], axis=1)
+    with _lib.Context(0) as c:
+        args = (st.atom_type, st.positions, st.cell, pairs, e.shift)
+        c.sk_assemble(_lib.SLOT_H, model.arrays("H"), *args)  # warm-up
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            nh = c.sk_assemble(_lib.SLOT_H, model.arrays("H"), *args)
+            ns = c.sk_assemble(_lib.SLOT_S, model.arrays("S"), *args)
+            best = min(best, time.perf_counter() - t0)
+    n_edges = int(e.src.size)
+    t0 = time.perf_counter()
+    for k in range(cpu_edges):
+        so.pair_block(model.table, "H", 0, 0, e.vec[k])
+    dt = time.perf_counter() - t0
+    gpu_rate = 2 * n_edges / best
+    cpu_rate = cpu_edges / dt
+    return {"workload": f"diamond Si {cells}^3 conventional cells ({st.n_atoms} atoms x 9 orbitals, s p d), "
+                        f"{n_edges} edges (16 neighbours), H and S, Gamma",
+            "gpu_ms_H_and_S": round(best * 1e3, 3), "stored_blocks": [nh, ns],
+            "value": gpu_rate, "unit": "SK edge blocks/s",
+            "cpu_baseline": {"value": cpu_rate, "unit": "SK edge blocks/s", "cores": 1, "kind": "port",
+                             "sample": f"first {cpu_edges} edges of H, Python oracle, {dt:.2f} s"},
+            "gpu_over_cpu": round(gpu_rate / cpu_rate, 1)}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -252,6 +318,8 @@ def main() -> int:
                         "dos_err_definition": "max_E |rho32 - rho64| / max_E rho64, Jackson, 2001 points"}
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(args.adj_cells, max(1, args.steps))
+    if not args.no_sk:
+        line["sk_assembly"] = sk_leg(args.sk_cells, 3, args.sk_cpu_edges)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(calc, args.cpu_moments, mu)
     print(json.dumps(line), flush=True)

@@ -95,6 +95,9 @@ struct rg_context {
     DevBuf partials, raw;
     // projected moments: rows sorted by group, chunk and group ranges, dots
     DevBuf pd_rows, pd_chunk, pd_grp, pd_partial, pd_out;
+    // Slater-Koster assembly scratch (kept between calls): table, atoms,
+    // edges, edge blocks, flags, gather plan
+    DevBuf sk_ints, sk_dbls, sk_type, sk_pos, sk_ei, sk_sh, sk_eblk, sk_flags, sk_sp, sk_src, sk_ph, sk_row;
 
     std::atomic<int> cancel{0};
 
@@ -647,6 +650,9 @@ void rg_destroy(rg_context *ctx) {
     ctx->raw.release();
     for (DevBuf *b : {&ctx->pd_rows, &ctx->pd_chunk, &ctx->pd_grp, &ctx->pd_partial, &ctx->pd_out})
         b->release();
+    for (DevBuf *b : {&ctx->sk_ints, &ctx->sk_dbls, &ctx->sk_type, &ctx->sk_pos, &ctx->sk_ei, &ctx->sk_sh,
+                      &ctx->sk_eblk, &ctx->sk_flags, &ctx->sk_sp, &ctx->sk_src, &ctx->sk_ph, &ctx->sk_row})
+        b->release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1002,6 +1008,8 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
         const int key = t->ch_row_type[order[q]] * T + t->ch_col_type[order[q]];
         int q1 = q;
         while (q1 < NC && t->ch_row_type[order[q1]] * T + t->ch_col_type[order[q1]] == key) ++q1;
+        if (q1 - q > sk_max_pair_channels())
+            return fail(ctx, RG_ERR_UNSUPPORTED, "sk table: too many channels for one type pair");
         pair_ch[2 * key] = q;
         pair_ch[2 * key + 1] = q1;
         q = q1;
@@ -1019,10 +1027,8 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
     ints.push_back(0); // keeps every array non-empty
     dbls.push_back(0.0);
 
-    DevBuf dints, ddbl, dtype, dpos, dei, dsh, deblk, dflags;
-    auto cleanup = [&]() {
-        for (DevBuf *x : {&dints, &ddbl, &dtype, &dpos, &dei, &dsh, &deblk, &dflags}) x->release();
-    };
+    DevBuf &dints = ctx->sk_ints, &ddbl = ctx->sk_dbls, &dtype = ctx->sk_type, &dpos = ctx->sk_pos;
+    DevBuf &dei = ctx->sk_ei, &dsh = ctx->sk_sh, &deblk = ctx->sk_eblk, &dflags = ctx->sk_flags;
     auto up = [&](DevBuf &d, const void *h, size_t bytes) -> hipError_t {
         hipError_t e = d.ensure(bytes ? bytes : 16);
         if (e == hipSuccess && bytes) e = hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
@@ -1036,10 +1042,7 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
     if (he == hipSuccess) he = up(dsh, edge_shift, (size_t)n_edges * 12);
     if (he == hipSuccess) he = deblk.ensure((size_t)std::max<int64_t>(n_edges, 1) * b * b * 8);
     if (he == hipSuccess) he = dflags.ensure((size_t)std::max<int64_t>(n_edges, 1));
-    if (he != hipSuccess) {
-        cleanup();
-        return hipfail(ctx, he, "sk assemble: upload");
-    }
+    if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: upload");
     SkDev sd;
     sd.n_types = T;
     sd.norb = b;
@@ -1078,26 +1081,31 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
     if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
     if (he == hipSuccess && n_edges)
         he = hipMemcpy(flags.data(), dflags.p, (size_t)n_edges, hipMemcpyDeviceToHost);
-    if (he != hipSuccess) {
-        cleanup();
-        return hipfail(ctx, he, "sk assemble: edge blocks");
-    }
+    if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: edge blocks");
 
     // pattern: (row, col, source) sorted; onsite (-1) first within a block,
     // then its edges in ascending order
     struct Src {
         int32_t row, col, src;
     };
-    std::vector<Src> srcs;
-    srcs.reserve((size_t)n_atoms + (size_t)n_edges);
-    for (int64_t a = 0; a < n_atoms; ++a) srcs.push_back({(int32_t)a, (int32_t)a, -1});
+    // counting sort by row (the onsite source first, edges in ascending
+    // order), then each row's few sources by (col, src)
+    std::vector<int64_t> rstart((size_t)n_atoms + 1, 0);
+    for (int64_t a = 0; a < n_atoms; ++a) rstart[a + 1] = 1;
     for (int64_t e = 0; e < n_edges; ++e)
-        if (flags[e]) srcs.push_back({edge_index[2 * e], edge_index[2 * e + 1], (int32_t)e});
-    std::sort(srcs.begin(), srcs.end(), [](const Src &x, const Src &y) {
-        if (x.row != y.row) return x.row < y.row;
-        if (x.col != y.col) return x.col < y.col;
-        return x.src < y.src;
-    });
+        if (flags[e]) ++rstart[edge_index[2 * e] + 1];
+    for (int64_t a = 0; a < n_atoms; ++a) rstart[a + 1] += rstart[a];
+    std::vector<Src> srcs((size_t)rstart[n_atoms]);
+    {
+        std::vector<int64_t> fill(rstart.begin(), rstart.end() - 1);
+        for (int64_t a = 0; a < n_atoms; ++a) srcs[fill[a]++] = {(int32_t)a, (int32_t)a, -1};
+        for (int64_t e = 0; e < n_edges; ++e)
+            if (flags[e]) srcs[fill[edge_index[2 * e]]++] = {edge_index[2 * e], edge_index[2 * e + 1], (int32_t)e};
+    }
+    for (int64_t a = 0; a < n_atoms; ++a)
+        std::sort(srcs.begin() + rstart[a], srcs.begin() + rstart[a + 1], [](const Src &x, const Src &y) {
+            return x.col != y.col ? x.col < y.col : x.src < y.src;
+        });
     std::vector<int64_t> row_ptr((size_t)n_atoms + 1, 0), src_ptr(1, 0);
     std::vector<int32_t> col_ind, src_id, row_of;
     std::vector<double> phase;
@@ -1125,7 +1133,7 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
     Slot &s = ctx->slots[slot];
     s.present = false;
     s.vals32_valid = false;
-    DevBuf dsp, dsrc, dph, drow;
+    DevBuf &dsp = ctx->sk_sp, &dsrc = ctx->sk_src, &dph = ctx->sk_ph, &drow = ctx->sk_row;
     const size_t nv = (size_t)nnzb * b * b * (cplx ? 2 : 1);
     he = up(dsp, src_ptr.data(), src_ptr.size() * 8);
     if (he == hipSuccess) he = up(dsrc, src_id.data(), src_id.size() * 4);
@@ -1144,8 +1152,6 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
                               static_cast<const double *>(deblk.p), static_cast<double *>(s.vals64.p), cplx,
                               ctx->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-    for (DevBuf *x : {&dsp, &dsrc, &dph, &drow}) x->release();
-    cleanup();
     if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: gather");
     s.nbr = n_atoms;
     s.b = b;

@@ -154,6 +154,8 @@ struct SkCell {
     double c[3][3];
 };
 hipError_t sk_upload_constants();
+// most channels one ordered type pair may carry on the device
+int sk_max_pair_channels();
 // every edge's row-major block into eblk[e b^2 ..], and flags[e] = any
 // |entry| > 1e-14
 hipError_t launch_sk_edge_blocks(const SkDev &t, int64_t n_edges, const int32_t *type, const double *pos,

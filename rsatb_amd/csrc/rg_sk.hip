@@ -1,15 +1,15 @@
 // Slater-Koster assembly of H or S on the device (rsatb/model/sktb/
 // sktb_construction.hpp:327-500, the reference's per-edge block evaluation).
 //
-// Two passes over the edge list, one thread per block element:
-//   k_sk_edge_blocks  evaluates every edge's b x b block (row-major) from the
-//                     bond vector: the table pair of the edge's atom types
-//                     if its cutoff covers the bond, else the transposed
-//                     block of the reversed pair at -vector (:476-500);
-//                     per channel the radial value times the two shell
-//                     rotations of the aligned sigma/pi/delta entries;
-//   k_sk_edge_flags   marks edges with an entry above 1e-14 (the reference
-//                     stores only those, :385);
+// Two kernels:
+//   k_sk_edges   one wave per edge: evaluates the edge's b x b block
+//                (row-major) from the bond vector (the table pair of the
+//                edge's atom types if its cutoff covers the bond, else the
+//                transposed block of the reversed pair at -vector,
+//                :476-500; per channel the radial value times the two shell
+//                rotations of the aligned sigma/pi/delta entries) and marks
+//                edges with an entry above 1e-14 (the reference stores only
+//                those, :385);
 // then, once the host has laid out the pattern of the kept blocks,
 //   k_sk_gather       sums each stored block's sources in a fixed order (the
 //                     onsite diagonal, then its edges in ascending order,
@@ -17,6 +17,8 @@
 //                     block layout.
 // All arithmetic is FP64; this is setup work, bound by the edge count.
 #include "rg_internal.h"
+
+#include <algorithm>
 
 namespace rg {
 namespace {
@@ -135,70 +137,91 @@ __device__ double radial(const SkDev &t, int c, double d) {
     return w * exp(-t.ch_lambda[c] * d) * poly;
 }
 
-__global__ __launch_bounds__(256) void k_sk_edge_blocks(SkDev t, int64_t n_edges, const int32_t *__restrict__ type,
-                                                        const double *__restrict__ pos, SkCell cell,
-                                                        const int32_t *__restrict__ edge_index,
-                                                        const int32_t *__restrict__ edge_shift,
-                                                        double *__restrict__ eblk) {
-    const int b = t.norb, bb = b * b;
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= n_edges * bb) return;
-    const int64_t e = tid / bb;
-    const int el = (int)(tid - e * bb), a = el / b, c = el - a * b;
-    const int i = edge_index[2 * e], j = edge_index[2 * e + 1];
-    const double s0 = edge_shift[3 * e], s1 = edge_shift[3 * e + 1], s2 = edge_shift[3 * e + 2];
-    double v[3];
-    for (int q = 0; q < 3; ++q)
-        v[q] = pos[3 * j + q] + s0 * cell.c[0][q] + s1 * cell.c[1][q] + s2 * cell.c[2][q] - pos[3 * i + q];
-    const double d = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    const int T = t.n_types;
-    int rt = type[i], ct = type[j];
-    int ra = a, ca = c;
-    double value = 0.0;
-    bool ok = d > 1e-12 && d <= t.pair_cutoff[rt * T + ct] + 1e-12;
-    if (!ok && d > 1e-12 && d <= t.pair_cutoff[ct * T + rt] + 1e-12) {
-        // reversed pair at -v, transposed back
-        const int tt = rt;
-        rt = ct;
-        ct = tt;
-        ra = c;
-        ca = a;
-        v[0] = -v[0];
-        v[1] = -v[1];
-        v[2] = -v[2];
-        ok = true;
-    }
-    if (ok) {
-        const Frame F = bond_frame(v[0], v[1], v[2]);
-        const int la = t.orb_l[rt * b + ra], ma = t.orb_m[rt * b + ra];
-        const int lc = t.orb_l[ct * b + ca], mc = t.orb_m[ct * b + ca];
-        const int c0 = t.pair_ch[2 * (rt * T + ct)], c1 = t.pair_ch[2 * (rt * T + ct) + 1];
-        for (int ch = c0; ch < c1; ++ch) {
-            if (t.ch_row_l[ch] != la || t.ch_col_l[ch] != lc) continue;
-            if (d > t.ch_r_cut[ch] + 1e-12) continue;
-            const double rv = radial(t, ch, d);
-            const double sign = (la > lc && ((la + lc) & 1)) ? -1.0 : 1.0;
-            const int m = t.ch_bond[ch];
-            double rot;
-            if (m == 0) {
-                rot = shell_rot(la, la, ma, F) * shell_rot(lc, lc, mc, F);
-            } else {
-                rot = shell_rot(la, la - m, ma, F) * shell_rot(lc, lc - m, mc, F) +
-                      shell_rot(la, la + m, ma, F) * shell_rot(lc, lc + m, mc, F);
-            }
-            value += sign * rv * rot;
-        }
-    }
-    eblk[tid] = value;
-}
+// One wave per edge (grid-stride). The wave first stages what every element
+// of the block shares: the 9 p and 25 d rotation entries of the bond frame
+// (lanes 0-33) and the radial value of each channel of the table pair (lanes
+// 34 on); then each lane sums the channel terms of its elements from LDS.
+constexpr int kSkWaves = 4;
+constexpr int kSkMaxPairCh = 30; // channels of one ordered type pair (s, p, d: at most 14)
 
-__global__ __launch_bounds__(256) void k_sk_edge_flags(int64_t n_edges, int bb, const double *__restrict__ eblk,
-                                                       uint8_t *__restrict__ flags) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_edges) return;
-    uint8_t f = 0;
-    for (int k = 0; k < bb; ++k) f |= fabs(eblk[e * bb + k]) > 1e-14 ? 1 : 0;
-    flags[e] = f;
+__global__ __launch_bounds__(64 * kSkWaves) void k_sk_edges(SkDev t, int64_t n_edges,
+                                                            const int32_t *__restrict__ type,
+                                                            const double *__restrict__ pos, SkCell cell,
+                                                            const int32_t *__restrict__ edge_index,
+                                                            const int32_t *__restrict__ edge_shift,
+                                                            double *__restrict__ eblk,
+                                                            uint8_t *__restrict__ flags) {
+    __shared__ double s_r1[kSkWaves][9], s_r2[kSkWaves][25], s_rad[kSkWaves][kSkMaxPairCh];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b = t.norb, bb = b * b, T = t.n_types;
+    for (int64_t e = (int64_t)blockIdx.x * kSkWaves + w; e < n_edges; e += (int64_t)gridDim.x * kSkWaves) {
+        const int i = edge_index[2 * e], j = edge_index[2 * e + 1];
+        const double s0 = edge_shift[3 * e], s1 = edge_shift[3 * e + 1], s2 = edge_shift[3 * e + 2];
+        double v[3];
+        for (int q = 0; q < 3; ++q)
+            v[q] = pos[3 * j + q] + s0 * cell.c[0][q] + s1 * cell.c[1][q] + s2 * cell.c[2][q] - pos[3 * i + q];
+        const double d = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        int rt = type[i], ct = type[j];
+        bool swap = false;
+        bool ok = d > 1e-12 && d <= t.pair_cutoff[rt * T + ct] + 1e-12;
+        if (!ok && d > 1e-12 && d <= t.pair_cutoff[ct * T + rt] + 1e-12) {
+            // reversed pair at -v, transposed back (sktb_construction.hpp:476-500)
+            swap = true;
+            const int tt = rt;
+            rt = ct;
+            ct = tt;
+            v[0] = -v[0];
+            v[1] = -v[1];
+            v[2] = -v[2];
+            ok = true;
+        }
+        double *out = eblk + e * bb;
+        if (!ok) {
+            for (int el = lane; el < bb; el += 64) out[el] = 0.0;
+            if (lane == 0) flags[e] = 0;
+            continue;
+        }
+        const int c0 = t.pair_ch[2 * (rt * T + ct)], nch = t.pair_ch[2 * (rt * T + ct) + 1] - c0;
+        {
+            const Frame F = bond_frame(v[0], v[1], v[2]);
+            if (lane < 9) {
+                s_r1[w][lane] = shell_rot(1, lane / 3, lane % 3, F);
+            } else if (lane < 34) {
+                const int q = lane - 9;
+                s_r2[w][q] = shell_rot(2, q / 5, q % 5, F);
+            } else if (lane - 34 < nch) {
+                const int ch = c0 + lane - 34;
+                s_rad[w][lane - 34] = d > t.ch_r_cut[ch] + 1e-12 ? 0.0 : radial(t, ch, d);
+            }
+        }
+        __builtin_amdgcn_wave_barrier(); // LDS operations of one wave complete in order
+        bool nz = false;
+        for (int el = lane; el < bb; el += 64) {
+            const int a = el / b, c = el - a * b;
+            const int ra = swap ? c : a, ca = swap ? a : c;
+            const int la = t.orb_l[rt * b + ra], ma = t.orb_m[rt * b + ra];
+            const int lc = t.orb_l[ct * b + ca], mc = t.orb_m[ct * b + ca];
+            const double sign = (la > lc && ((la + lc) & 1)) ? -1.0 : 1.0;
+            auto rot = [&](int l, int loc, int glo) -> double {
+                return l == 0 ? 1.0 : (l == 1 ? s_r1[w][loc * 3 + glo] : s_r2[w][loc * 5 + glo]);
+            };
+            double value = 0.0;
+            for (int k = 0; k < nch; ++k) {
+                const int ch = c0 + k;
+                if (t.ch_row_l[ch] != la || t.ch_col_l[ch] != lc) continue;
+                const int m = t.ch_bond[ch];
+                const double r = m == 0 ? rot(la, la, ma) * rot(lc, lc, mc)
+                                        : rot(la, la - m, ma) * rot(lc, lc - m, mc) +
+                                              rot(la, la + m, ma) * rot(lc, lc + m, mc);
+                value += sign * s_rad[w][k] * r;
+            }
+            out[el] = value;
+            nz = nz || fabs(value) > 1e-14;
+        }
+        const uint64_t any = __ballot(nz);
+        if (lane == 0) flags[e] = any != 0;
+        __builtin_amdgcn_wave_barrier(); // staged values are read before the next edge's writes
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sk_gather(int64_t nnzb, int b, const int64_t *__restrict__ src_ptr,
@@ -250,16 +273,14 @@ hipError_t sk_upload_constants() {
 hipError_t launch_sk_edge_blocks(const SkDev &t, int64_t n_edges, const int32_t *type, const double *pos,
                                  const SkCell &cell, const int32_t *edge_index, const int32_t *edge_shift,
                                  double *eblk, uint8_t *flags, hipStream_t st) {
-    const int64_t n = n_edges * (int64_t)t.norb * t.norb;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sk_edge_blocks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, n_edges, type,
-                       pos, cell, edge_index, edge_shift, eblk);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sk_edge_flags, dim3((unsigned)((n_edges + 255) / 256)), dim3(256), 0, st, n_edges,
-                       t.norb * t.norb, eblk, flags);
+    if (n_edges == 0) return hipSuccess;
+    const int64_t grid = std::min<int64_t>((n_edges + kSkWaves - 1) / kSkWaves, 16384);
+    hipLaunchKernelGGL(k_sk_edges, dim3((unsigned)grid), dim3(64 * kSkWaves), 0, st, t, n_edges, type, pos, cell,
+                       edge_index, edge_shift, eblk, flags);
     return hipGetLastError();
 }
+
+int sk_max_pair_channels() { return kSkMaxPairCh; }
 
 hipError_t launch_sk_gather(int64_t nnzb, int b, const int64_t *src_ptr, const int32_t *src, const double2 *phase,
                             const int32_t *row_of, const int32_t *type, const double *onsite, const double *eblk,

@@ -143,10 +143,16 @@ class RSATB:
                 "PreconditionerOptions.maxiter > 0 asks for the sparse Newton-Schulz refinement, which the GPU "
                 "path does not provide; the factor is the graph-local seed (maxiter = 0)."
             )
+        device_sk = getattr(model, "device_assembly", False)
         for k in o.kpoints:
+            if device_sk:
+                # built on the device when the k-point is bound (rg_sk_assemble)
+                self._kstates.append({"k": tuple(k), "H": None, "S": None, "Z": None, "sk": True,
+                                      "has_S": sandwich})
+                continue
             H = assemble(st, self.edges, model, "H", k)
             S = assemble(st, self.edges, model, "S", k) if sandwich else None
-            self._kstates.append({"k": tuple(k), "H": H, "S": S, "Z": None})
+            self._kstates.append({"k": tuple(k), "H": H, "S": S, "Z": None, "has_S": sandwich})
         self._bound_k = None
         if o.energy_bounds is not None:
             self._bounds = make_explicit_bounds(*o.energy_bounds)
@@ -167,20 +173,31 @@ class RSATB:
     bounds_padding = 0.5
 
     def _form(self, idx: int) -> int:
-        return _lib.FORM_SANDWICH if self._kstates[idx]["S"] is not None else _lib.FORM_ORTHOGONAL
+        return _lib.FORM_SANDWICH if self._kstates[idx]["has_S"] else _lib.FORM_ORTHOGONAL
+
+    def _sk_assemble(self, slot: int, which: str, k) -> None:
+        st, e = self.structure, self.edges
+        self._ctx.sk_assemble(slot, self.model.arrays(which), st.atom_type, st.positions, st.cell,
+                              np.stack([e.src, e.dst], axis=1), e.shift, None if not np.any(k) else k)
 
     def _bind_factors(self, idx: int) -> None:
         """Puts H(k) (and Z, Z^H for an overlap) on the device."""
         if self._bound_k == idx:
             return
         ks = self._kstates[idx]
-        self._ctx.upload_bsr(_lib.SLOT_H, ks["H"])
-        if ks["S"] is not None:
+        if ks.get("sk"):
+            self._sk_assemble(_lib.SLOT_H, "H", ks["k"])
+        else:
+            self._ctx.upload_bsr(_lib.SLOT_H, ks["H"])
+        if ks["has_S"]:
             if ks["Z"] is not None:
                 self._ctx.upload_bsr(_lib.SLOT_Z, ks["Z"])
                 self._ctx.upload_bsr(_lib.SLOT_ZH, ks["Z"].conj_transpose())
             else:
-                self._ctx.upload_bsr(_lib.SLOT_S, ks["S"])
+                if ks.get("sk"):
+                    self._sk_assemble(_lib.SLOT_S, "S", ks["k"])
+                else:
+                    self._ctx.upload_bsr(_lib.SLOT_S, ks["S"])
                 ks["isqrt"] = self._ctx.isqrt_patches()
         self._bound_k = idx
 
@@ -201,7 +218,7 @@ class RSATB:
     def overlap_factor(self, k_idx: int = 0):
         """The factor Z of a k-point as the device holds it (downloaded)."""
         self._bind_factors(k_idx)
-        if self._kstates[k_idx]["S"] is None:
+        if not self._kstates[k_idx]["has_S"]:
             raise RuntimeError("this model has no overlap")
         return self._ctx.download_bsr(_lib.SLOT_Z)
 
@@ -280,6 +297,12 @@ class RSATB:
         M = ks.get(which)
         if which not in ("H", "S"):
             raise RuntimeError(f"unknown operator name: {which}")
+        if ks.get("sk") and (which == "H" or ks["has_S"]):
+            # assembled on the device: build it into its slot and read it back
+            slot = _lib.SLOT_H if which == "H" else _lib.SLOT_S
+            self._sk_assemble(slot, which, ks["k"])
+            self._bound_k = None
+            return self._ctx.download_bsr(slot).to_coo()
         if M is None:
             raise RuntimeError(f"{which} is not populated for this model")
         return M.to_coo()

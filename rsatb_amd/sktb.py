@@ -374,3 +374,34 @@ class SkTable:
             out["ch_" + k] = np.array(cols[k], dtype=np.float64)
         out["n_channels"] = len(cols["row_type"])
         return out
+
+
+class SktbModel:
+    """Slater-Koster tight-binding model whose H and S are assembled on the
+    device (SktbModel of the reference, rsatb/model/sktb/sktb_model.hpp, over
+    SkTableConstruction). Structure atom types index the table's types, which
+    are ordered by atomic number (sk_table.hpp:289-298)."""
+
+    device_assembly = True
+
+    def __init__(self, basis: dict, hamiltonian: SkParameters, overlap: SkParameters | None = None):
+        self.table = SkTable(basis, hamiltonian, overlap)
+        norb = self.table.type_norb
+        if len(set(norb)) != 1:
+            raise ValueError("the device BSR holds uniform blocks: every atom type needs the same orbital count")
+        self.norb = norb[0]
+        self._arrays = {"H": self.table.arrays("H")}
+        if self.table.has_overlap:
+            self._arrays["S"] = self.table.arrays("S")
+
+    @property
+    def cutoff(self) -> float:
+        """Neighbour-list cutoff: the largest bond cutoff of any pair (each
+        pair's r_i + r_j of sk_table.hpp:176-191 is at most this)."""
+        return max((max(p.h_max_cutoff, p.s_max_cutoff) for p in self.table.pairs.values()), default=0.0)
+
+    def has_overlap(self) -> bool:
+        return self.table.has_overlap
+
+    def arrays(self, which: str) -> dict:
+        return self._arrays[which]

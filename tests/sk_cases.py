@@ -6,8 +6,9 @@ import numpy as np
 from rsatb_amd.sktb import Radial, SkParameters, SkTable
 
 
-def sp3_silicon_table(exp_poly=True):
-    """An sp3d5 silicon-like model with every channel; radial kinds mixed."""
+def silicon_params(exp_poly=True):
+    """(basis, H, S) of an sp3d5 silicon-like model with every channel;
+    radial kinds mixed."""
     h = SkParameters()
     h.onsite["Si"] = {"s": -4.2, "p": 1.7, "d": 6.5}
     vals = {("ss", "sigma"): -1.9, ("sp", "sigma"): 2.1, ("pp", "sigma"): 3.0, ("pp", "pi"): -0.9,
@@ -24,7 +25,11 @@ def sp3_silicon_table(exp_poly=True):
     for i, ((sp, bond), v) in enumerate(vals.items()):
         r = np.linspace(1.8, 3.2, 6)
         s.set_pair_channel("Si", "Si", sp, bond, Radial.spline(r, 0.03 * v * np.exp(-(r - 2.35)), 3.2))
-    return SkTable({"Si": ["s", "p", "d"]}, h, s)
+    return {"Si": ["s", "p", "d"]}, h, s
+
+
+def sp3_silicon_table(exp_poly=True):
+    return SkTable(*silicon_params(exp_poly))
 
 
 def periodic_edges(pos, cell, cutoff):

@@ -140,3 +140,37 @@ def test_invalid_inputs_rejected(ctx):
     with pytest.raises(_lib.RsatbGpuError, match="atom type"):
         ctx.sk_assemble(_lib.SLOT_H, a, np.full(8, 3, np.int32), pos, cell, np.array([[0, 1]]),
                         np.zeros((1, 3), np.int64))
+
+
+def test_driver_sktb_model_sandwich_dos():
+    """RSATB over an SK model: H and S assembled on the device, Z from S,
+    then the DOS task; checked against the oracle's assembly and the moment
+    oracle with the device's Z."""
+    from oracle import oracle as O
+    from rsatb_amd.bsr import BSR
+    from rsatb_amd.driver import PreconditionerOptions, RSATB, SetupOptions
+    from rsatb_amd.lattice import Structure
+    from rsatb_amd.sktb import SktbModel
+    from sk_cases import silicon_params
+
+    pos, cell = silicon_cell()
+    st = Structure(pos, cell)
+    model = SktbModel(*silicon_params())
+    calc = RSATB(st, model, SetupOptions(nsamples=8, compute_bounds=True,
+                                         preconditioner=PreconditionerOptions(operator_form="sandwich")))
+    e = calc.edges
+    ei = np.stack([e.src, e.dst])
+    rp, ci, blk = so.assemble(model.table, "H", st.atom_type, pos, cell, ei, e.shift)
+    rows, cols, vals = calc.dump_operator_coo("H")
+    dense = np.zeros((len(pos) * 9, len(pos) * 9))
+    dense[rows, cols] = vals
+    ref = BSR(len(pos), 9, rp, ci, blk).to_dense()
+    assert np.max(np.abs(dense - ref)) < 1e-12
+    emin, emax = calc.energy_bounds()
+    dos, omega = calc.compute_dos(dE=0.25)
+    Z = calc.overlap_factor()
+    psi = calc.context.get_samples()
+    op = O.Operator(O.FORM_SANDWICH, BSR(len(pos), 9, rp, ci, blk), (emax + emin) / 2, (emax - emin) / 2, Z)
+    rdos, romega = O.compute_dos(op, psi, emin, emax, 0.25)
+    assert np.allclose(omega, romega, rtol=0, atol=1e-12)
+    assert np.max(np.abs(dos - rdos)) <= 1e-10 * np.max(np.abs(rdos))
