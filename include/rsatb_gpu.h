@@ -62,6 +62,12 @@ typedef struct rg_context rg_context;
 #define RG_SLOT_Z 1
 #define RG_SLOT_ZH 2
 #define RG_SLOT_S 3 /* overlap input of rg_isqrt_patches */
+/* bond-moment velocity factors B_a (V_a = i B_a) of rg_sk_velocity, read by
+ * rg_kubo_greenwood */
+#define RG_SLOT_VX 4
+#define RG_SLOT_VY 5
+#define RG_SLOT_VZ 6
+#define RG_SLOT_COUNT 7
 
 /* value types of an uploaded matrix */
 #define RG_F64 0
@@ -235,6 +241,31 @@ RG_API int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *table, i
                           const int32_t *atom_type, const double *pos, const double *cell,
                           int64_t n_edges, const int32_t *edge_index, const int32_t *edge_shift,
                           const double *kpoint, int64_t *nnzb_out);
+
+/* ---- Kubo-Greenwood DC conductivity (dc_spectrum_fftcheb, Greenwood formulation,
+ *      rsatb/calc/fftcheb_transport.hpp:543-623; velocity of an orthogonal basis,
+ *      rsatb/operator/commutator_velocity.hpp:7-43) ---- */
+
+/* The commutator velocity V_a = i B_a of an orthogonal SK model at Gamma:
+ * B_a = sum over edges of (pos_j + shift . cell - pos_i)_a times the edge's
+ * SK block (AccumulateBondMoment with the displacement weight and factor i),
+ * into RG_SLOT_VX .. RG_SLOT_VZ (real, H's stored pattern). Same arguments as
+ * rg_sk_assemble with the Hamiltonian table. */
+RG_API int rg_sk_velocity(rg_context *ctx, const rg_sk_table *table, int64_t n_atoms,
+                          const int32_t *atom_type, const double *pos, const double *cell,
+                          int64_t n_edges, const int32_t *edge_index, const int32_t *edge_shift);
+
+/* Greenwood conductivity spectrum of the orthogonal form (H in RG_SLOT_H,
+ * velocity factors in RG_SLOT_VX.., window of rg_set_operator, samples of
+ * rg_set_samples) with M moments and damping `kernel` (RG_KERNEL_*):
+ * sigma_out[k * 3 + a] = prefactor * kernel_aa(eps) on the ascending
+ * Chebyshev nodes, energy_out[k] their energies (may be NULL). The left
+ * and right Chebyshev sequences of every sample are kept on the device and
+ * contracted as mu^a = L^H R (one GEMM per axis); mu_out (may be NULL)
+ * receives [3][M][M] complex moments scaled by 1 / (2 a)^2 and summed over
+ * samples. Memory: 6 M x rows x R vector elements. */
+RG_API int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, double *sigma_out,
+                             double *energy_out, double *mu_out);
 
 /* ---- graph-local S^{-1/2} (graph_inverse_sqrt seed, rsatb/calc/isqrtm.hpp:137-232;
  *      ISQRTMPreconditioner ctor, rsatb/calc/preconditioner.hpp:184-192) ---- */

@@ -22,6 +22,7 @@ RG_ERR_NOMEM = 5
 RG_ERR_UNSUPPORTED = 6
 
 SLOT_H, SLOT_Z, SLOT_ZH, SLOT_S = 0, 1, 2, 3
+SLOT_VX, SLOT_VY, SLOT_VZ = 4, 5, 6
 F64, C128 = 0, 2
 FORM_ORTHOGONAL, FORM_SANDWICH = 0, 1
 SAMPLES_RADEMACHER, SAMPLES_PHASE, SAMPLES_UPLOAD_REAL, SAMPLES_UPLOAD_COMPLEX = 0, 1, 2, 3
@@ -72,6 +73,8 @@ SIGNATURES = {
     "rg_download_bsr": (_i, [_vp, _i, _pi64, _pi32, _pd]),
     "rg_sk_assemble": (_i, [_vp, _i, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32, _pd,
                             _pi64]),
+    "rg_sk_velocity": (_i, [_vp, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32]),
+    "rg_kubo_greenwood": (_i, [_vp, _i, _i, _d, _pd, _pd, _pd]),
     "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_set_checkpointing": (_i, [_vp, _i]),
@@ -217,6 +220,28 @@ class Context:
         """Builds H(k) or S(k) of an SK table on the device into `slot`
         (table_arrays: rsatb_amd.sktb.SkTable.arrays(which); edge_index as
         flattened (src, dst) pairs). Returns the stored block count."""
+        return self._sk_call(False, slot, table_arrays, atom_type, pos, cell, edge_index, edge_shift, kpoint)
+
+    def sk_velocity(self, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift) -> None:
+        """Bond-moment velocity factors B_x, B_y, B_z (V = i B) of an
+        orthogonal SK model at Gamma into SLOT_VX .. SLOT_VZ."""
+        self._sk_call(True, SLOT_VX, table_arrays, atom_type, pos, cell, edge_index, edge_shift, None)
+
+    def kubo_greenwood(self, n_moments: int, kernel: int = KERNEL_JACKSON, prefactor: float = 1.0,
+                       with_moments: bool = False):
+        """(sigma [M, 3] for xx, yy, zz on ascending nodes, energies [M]
+        [, mu [3, M, M] complex])."""
+        M = int(n_moments)
+        sigma = np.zeros((M, 3))
+        energy = np.zeros(M)
+        mu = np.zeros((3, M, M), dtype=np.complex128) if with_moments else None
+        rc = self._lib.rg_kubo_greenwood(self._h, M, int(kernel), float(prefactor), _dptr(sigma), _dptr(energy),
+                                         _dptr(mu.view(np.float64)) if with_moments else None)
+        _check(self._h, rc, "rg_kubo_greenwood")
+        return (sigma, energy, mu) if with_moments else (sigma, energy)
+
+    def _sk_call(self, velocity: bool, slot: int, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift,
+                 kpoint):
         a = table_arrays
         keep = {}
 
@@ -242,6 +267,12 @@ class Context:
             raise ValueError("edge_index needs (src, dst) pairs and edge_shift three integers per edge")
         if pos.shape[0] != types.size:
             raise ValueError("pos and atom_type describe different atom counts")
+        if velocity:
+            rc = self._lib.rg_sk_velocity(self._h, C.byref(t), int(types.size), types.ctypes.data_as(_pi32),
+                                          _dptr(pos), _dptr(cell), int(ei.size // 2), ei.ctypes.data_as(_pi32),
+                                          sh.ctypes.data_as(_pi32))
+            _check(self._h, rc, "rg_sk_velocity")
+            return None
         kp = None if kpoint is None else np.ascontiguousarray(kpoint, dtype=np.float64).reshape(3)
         nnzb = C.c_int64()
         rc = self._lib.rg_sk_assemble(self._h, int(slot), C.byref(t), int(types.size), types.ctypes.data_as(_pi32),

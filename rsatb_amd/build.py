@@ -64,7 +64,7 @@ def build(verbose: bool = False, jobs: int = 4) -> Path:
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
     tmp = LIB.with_suffix(".so.tmp")
-    link = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    link = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp), "-lrocblas"]
     if verbose:
         print(" ".join(link), flush=True)
     r = subprocess.run(link, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

@@ -10,6 +10,8 @@
 #include "../../include/rsatb_gpu.h"
 #include "rg_internal.h"
 
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -79,7 +81,7 @@ struct rg_context {
     hipStream_t stream = nullptr;
     std::string err;
 
-    Slot slots[4];
+    Slot slots[RG_SLOT_COUNT];
 
     bool op_set = false;
     int form = RG_FORM_ORTHOGONAL;
@@ -98,6 +100,9 @@ struct rg_context {
     // Slater-Koster assembly scratch (kept between calls): table, atoms,
     // edges, edge blocks, flags, gather plan
     DevBuf sk_ints, sk_dbls, sk_type, sk_pos, sk_ei, sk_sh, sk_eblk, sk_flags, sk_sp, sk_src, sk_ph, sk_row;
+    // Kubo-Greenwood: stored left/right sequences and the moment matrices
+    DevBuf kg_store, kg_mu;
+    rocblas_handle blas = nullptr;
 
     std::atomic<int> cancel{0};
 
@@ -651,8 +656,10 @@ void rg_destroy(rg_context *ctx) {
     for (DevBuf *b : {&ctx->pd_rows, &ctx->pd_chunk, &ctx->pd_grp, &ctx->pd_partial, &ctx->pd_out})
         b->release();
     for (DevBuf *b : {&ctx->sk_ints, &ctx->sk_dbls, &ctx->sk_type, &ctx->sk_pos, &ctx->sk_ei, &ctx->sk_sh,
-                      &ctx->sk_eblk, &ctx->sk_flags, &ctx->sk_sp, &ctx->sk_src, &ctx->sk_ph, &ctx->sk_row})
+                      &ctx->sk_eblk, &ctx->sk_flags, &ctx->sk_sp, &ctx->sk_src, &ctx->sk_ph, &ctx->sk_row,
+                      &ctx->kg_store, &ctx->kg_mu})
         b->release();
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -662,7 +669,7 @@ const char *rg_last_error(const rg_context *ctx) { return ctx ? ctx->err.c_str()
 int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *row_ptr,
                   const int32_t *col_ind, const void *values, int dtype) {
     if (!ctx) return RG_ERR_INVALID;
-    if (slot < 0 || slot > 3) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
+    if (slot < 0 || slot >= RG_SLOT_COUNT) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
     if (nbr < 1 || b < 1 || b > 16) return fail(ctx, RG_ERR_INVALID, "block rows >= 1 and 1 <= b <= 16");
     if (!row_ptr || !col_ind || !values) return fail(ctx, RG_ERR_INVALID, "null array");
     if (dtype != RG_F64 && dtype != RG_C128)
@@ -761,7 +768,7 @@ int rg_get_samples(rg_context *ctx, double *host_out, int as_complex) {
 
 int rg_bsr_mult(rg_context *ctx, int slot, int R, const double *x, double *y, int is_complex) {
     if (!ctx || !x || !y) return RG_ERR_INVALID;
-    if (slot < 0 || slot > 3 || !ctx->slots[slot].present)
+    if (slot < 0 || slot >= RG_SLOT_COUNT || !ctx->slots[slot].present)
         return fail(ctx, RG_ERR_STATE, "matrix slot is empty");
     if (R < 1 || R > 256) return fail(ctx, RG_ERR_INVALID, "1 <= R <= 256");
     RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
@@ -916,7 +923,7 @@ int rg_kpm_pdos_moments(rg_context *ctx, int M, int precision, int doubling, int
 int rg_bsr_info(rg_context *ctx, int slot, int64_t *n_block_rows, int *block_dim, int64_t *nnzb,
                 int *is_complex) {
     if (!ctx) return RG_ERR_INVALID;
-    if (slot < 0 || slot > 3 || !ctx->slots[slot].present)
+    if (slot < 0 || slot >= RG_SLOT_COUNT || !ctx->slots[slot].present)
         return fail(ctx, RG_ERR_STATE, "matrix slot is empty");
     const Slot &s = ctx->slots[slot];
     if (n_block_rows) *n_block_rows = s.nbr;
@@ -928,7 +935,7 @@ int rg_bsr_info(rg_context *ctx, int slot, int64_t *n_block_rows, int *block_dim
 
 int rg_download_bsr(rg_context *ctx, int slot, int64_t *row_ptr, int32_t *col_ind, double *values) {
     if (!ctx) return RG_ERR_INVALID;
-    if (slot < 0 || slot > 3 || !ctx->slots[slot].present)
+    if (slot < 0 || slot >= RG_SLOT_COUNT || !ctx->slots[slot].present)
         return fail(ctx, RG_ERR_STATE, "matrix slot is empty");
     RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     const Slot &s = ctx->slots[slot];
@@ -951,11 +958,43 @@ int rg_download_bsr(rg_context *ctx, int slot, int64_t *row_ptr, int32_t *col_in
     return RG_OK;
 }
 
-int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, const int32_t *atom_type,
-                   const double *pos, const double *cell, int64_t n_edges, const int32_t *edge_index,
-                   const int32_t *edge_shift, const double *kpoint, int64_t *nnzb_out) {
+} // extern "C"
+
+namespace {
+
+// KPM damping weights g_m (kpm_kernel_weights, rsatb/calc/kpm.hpp:52-85)
+int kpm_weights(int kernel, int M, std::vector<double> &gk) {
+    gk.assign((size_t)M, 1.0);
+    const int n0 = M / 2;
+    switch (kernel) {
+    case RG_KERNEL_NONE: break;
+    case RG_KERNEL_JACKSON: {
+        const double denom = (double)(M + 1);
+        const double theta = M_PI / denom;
+        const double cot_theta = std::cos(theta) / std::sin(theta);
+        for (int i = 1; i < M; ++i)
+            gk[i] = ((denom - i) * std::cos(theta * i) + cot_theta * std::sin(theta * i)) / denom;
+        break;
+    }
+    case RG_KERNEL_COSTAPER:
+        for (int i = n0; i < M; ++i) gk[i] = 0.5 * (1.0 + std::cos(M_PI * (double)(i - n0) / (double)(M - n0)));
+        break;
+    case RG_KERNEL_VPTAPER:
+        for (int i = n0; i < M; ++i) gk[i] = (double)(M - i) / (double)(M - n0);
+        break;
+    default: return RG_ERR_INVALID;
+    }
+    return RG_OK;
+}
+
+// SK assembly behind rg_sk_assemble (velocity = false: H(k) or S(k) into
+// `slot`) and rg_sk_velocity (velocity = true: the bond moments B_a = sum over
+// edges of (bond)_a times the edge block, at Gamma, into RG_SLOT_VX + a).
+int sk_build(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, const int32_t *atom_type,
+             const double *pos, const double *cell, int64_t n_edges, const int32_t *edge_index,
+             const int32_t *edge_shift, const double *kpoint, bool velocity, int64_t *nnzb_out) {
     if (!ctx) return RG_ERR_INVALID;
-    if (slot < 0 || slot > 3) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
+    if (slot < 0 || slot >= RG_SLOT_COUNT) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
     if (!t || !atom_type || !pos || !cell) return fail(ctx, RG_ERR_INVALID, "null array");
     if (n_edges < 0 || (n_edges > 0 && (!edge_index || !edge_shift)))
         return fail(ctx, RG_ERR_INVALID, "edge arrays missing");
@@ -990,6 +1029,7 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
         if (edge_index[e] < 0 || edge_index[e] >= n_atoms)
             return fail(ctx, RG_ERR_INVALID, "edge atom index out of range");
     const bool cplx = kpoint && (kpoint[0] != 0.0 || kpoint[1] != 0.0 || kpoint[2] != 0.0);
+    if (velocity && cplx) return fail(ctx, RG_ERR_UNSUPPORTED, "sk velocity: Gamma point only");
     RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     RG_HIP(ctx, sk_upload_constants(), "SK constants");
 
@@ -1109,6 +1149,7 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
     std::vector<int64_t> row_ptr((size_t)n_atoms + 1, 0), src_ptr(1, 0);
     std::vector<int32_t> col_ind, src_id, row_of;
     std::vector<double> phase;
+    std::vector<double> bond[3]; // per source, velocity only
     for (size_t q = 0; q < srcs.size(); ++q) {
         const bool first = q == 0 || srcs[q].row != srcs[q - 1].row || srcs[q].col != srcs[q - 1].col;
         if (first) {
@@ -1118,6 +1159,19 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
             ++row_ptr[srcs[q].row + 1];
         }
         src_id.push_back(srcs[q].src);
+        if (velocity) {
+            const int32_t e = srcs[q].src;
+            for (int a = 0; a < 3; ++a) {
+                double w = 0.0; // the onsite block has no bond
+                if (e >= 0) {
+                    const int i = edge_index[2 * (int64_t)e], j = edge_index[2 * (int64_t)e + 1];
+                    w = pos[3 * j + a] - pos[3 * i + a];
+                    for (int r = 0; r < 3; ++r) w += edge_shift[3 * (int64_t)e + r] * cell[3 * r + a];
+                }
+                bond[a].push_back(w);
+                bond[a].push_back(0.0);
+            }
+        }
         if (cplx) {
             double arg = 0.0;
             if (srcs[q].src >= 0)
@@ -1130,37 +1184,222 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
     for (int64_t a = 0; a < n_atoms; ++a) row_ptr[a + 1] += row_ptr[a];
     const int64_t nnzb = (int64_t)col_ind.size();
 
-    Slot &s = ctx->slots[slot];
-    s.present = false;
-    s.vals32_valid = false;
     DevBuf &dsp = ctx->sk_sp, &dsrc = ctx->sk_src, &dph = ctx->sk_ph, &drow = ctx->sk_row;
     const size_t nv = (size_t)nnzb * b * b * (cplx ? 2 : 1);
     he = up(dsp, src_ptr.data(), src_ptr.size() * 8);
     if (he == hipSuccess) he = up(dsrc, src_id.data(), src_id.size() * 4);
-    if (he == hipSuccess) he = up(dph, phase.data(), phase.size() * 8);
     if (he == hipSuccess) he = up(drow, row_of.data(), row_of.size() * 4);
-    if (he == hipSuccess) he = s.row_ptr.ensure((size_t)(n_atoms + 1) * 8);
-    if (he == hipSuccess) he = s.col_ind.ensure((size_t)nnzb * 4 + 4);
-    if (he == hipSuccess) he = s.vals64.ensure(nv * 8);
-    if (he == hipSuccess)
-        he = hipMemcpy(s.row_ptr.p, row_ptr.data(), (size_t)(n_atoms + 1) * 8, hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(s.col_ind.p, col_ind.data(), (size_t)nnzb * 4, hipMemcpyHostToDevice);
-    if (he == hipSuccess)
-        he = launch_sk_gather(nnzb, b, static_cast<const int64_t *>(dsp.p), static_cast<const int32_t *>(dsrc.p),
-                              static_cast<const double2 *>(dph.p), static_cast<const int32_t *>(drow.p),
-                              static_cast<const int32_t *>(dtype.p), sd.onsite,
-                              static_cast<const double *>(deblk.p), static_cast<double *>(s.vals64.p), cplx,
-                              ctx->stream);
-    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-    if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: gather");
-    s.nbr = n_atoms;
-    s.b = b;
-    s.nnzb = nnzb;
-    s.is_complex = cplx;
-    int rcg = build_xrow(ctx, s, col_ind.data());
-    if (rcg) return rcg;
-    s.present = true;
+    if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: plan upload");
+    const int nout = velocity ? 3 : 1;
+    for (int o = 0; o < nout; ++o) {
+        const int sl = velocity ? RG_SLOT_VX + o : slot;
+        const std::vector<double> &w = velocity ? bond[o] : phase;
+        Slot &s = ctx->slots[sl];
+        s.present = false;
+        s.vals32_valid = false;
+        he = up(dph, w.data(), w.size() * 8);
+        if (he == hipSuccess) he = s.row_ptr.ensure((size_t)(n_atoms + 1) * 8);
+        if (he == hipSuccess) he = s.col_ind.ensure((size_t)nnzb * 4 + 4);
+        if (he == hipSuccess) he = s.vals64.ensure(nv * 8);
+        if (he == hipSuccess)
+            he = hipMemcpy(s.row_ptr.p, row_ptr.data(), (size_t)(n_atoms + 1) * 8, hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(s.col_ind.p, col_ind.data(), (size_t)nnzb * 4, hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = launch_sk_gather(nnzb, b, static_cast<const int64_t *>(dsp.p),
+                                  static_cast<const int32_t *>(dsrc.p), static_cast<const double2 *>(dph.p),
+                                  static_cast<const int32_t *>(drow.p), static_cast<const int32_t *>(dtype.p),
+                                  sd.onsite, static_cast<const double *>(deblk.p),
+                                  static_cast<double *>(s.vals64.p), velocity ? 2 : (cplx ? 1 : 0), ctx->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+        if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: gather");
+        s.nbr = n_atoms;
+        s.b = b;
+        s.nnzb = nnzb;
+        s.is_complex = cplx;
+        int rcg = build_xrow(ctx, s, col_ind.data());
+        if (rcg) return rcg;
+        s.present = true;
+    }
     if (nnzb_out) *nnzb_out = nnzb;
+    return RG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, const int32_t *atom_type,
+                   const double *pos, const double *cell, int64_t n_edges, const int32_t *edge_index,
+                   const int32_t *edge_shift, const double *kpoint, int64_t *nnzb_out) {
+    return sk_build(ctx, slot, t, n_atoms, atom_type, pos, cell, n_edges, edge_index, edge_shift, kpoint, false,
+                    nnzb_out);
+}
+
+int rg_sk_velocity(rg_context *ctx, const rg_sk_table *t, int64_t n_atoms, const int32_t *atom_type,
+                   const double *pos, const double *cell, int64_t n_edges, const int32_t *edge_index,
+                   const int32_t *edge_shift) {
+    return sk_build(ctx, RG_SLOT_VX, t, n_atoms, atom_type, pos, cell, n_edges, edge_index, edge_shift, nullptr,
+                    true, nullptr);
+}
+
+int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, double *sigma_out,
+                      double *energy_out, double *mu_out) {
+    if (!ctx || !sigma_out) return RG_ERR_INVALID;
+    int rc = check_operator(ctx);
+    if (rc) return rc;
+    if (ctx->form != RG_FORM_ORTHOGONAL)
+        return fail(ctx, RG_ERR_UNSUPPORTED, "kubo-greenwood: the orthogonal form only (an overlap needs the "
+                                             "composite velocity)");
+    if (!ctx->samples_set) return fail(ctx, RG_ERR_STATE, "samples not set (rg_set_samples)");
+    if (M < 1) return fail(ctx, RG_ERR_INVALID, "n_moments must be >= 1");
+    std::vector<double> gk;
+    if (kpm_weights(kernel, M, gk) != RG_OK) return fail(ctx, RG_ERR_INVALID, "unknown kernel");
+    const Slot &H = ctx->slots[RG_SLOT_H];
+    for (int a = 0; a < 3; ++a) {
+        const Slot &V = ctx->slots[RG_SLOT_VX + a];
+        if (!V.present) return fail(ctx, RG_ERR_STATE, "velocity slots are empty (rg_sk_velocity)");
+        if (V.nbr != H.nbr || V.b != H.b) return fail(ctx, RG_ERR_INVALID, "velocity and H differ in shape");
+        if (V.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "velocity factors must be real (Gamma)");
+    }
+    const int64_t rows = op_rows(ctx);
+    if (ctx->n_rows != rows) return fail(ctx, RG_ERR_INVALID, "sample rows differ from operator rows");
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->cancel.store(0);
+    const int R = ctx->R;
+    VecMode vm;
+    vm.complex_vec = vec_complex(ctx);
+    const size_t eb = vec_elem_bytes(vm), K = (size_t)rows * R, nb = K * eb;
+    for (int i = 0; i < 3; ++i) RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
+    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
+    // [left | right][axis][moment] vectors of K elements
+    RG_HIP(ctx, ctx->kg_store.ensure(6 * (size_t)M * nb), "hipMalloc sequence store");
+    char *store = static_cast<char *>(ctx->kg_store.p);
+    auto Lst = [&](int a, int m) { return (void *)(store + ((size_t)a * M + m) * nb); };
+    auto Rst = [&](int a, int m) { return (void *)(store + ((size_t)(3 + a) * M + m) * nb); };
+
+    const double ha = ctx->half_width, c = ctx->center;
+    EpiArgs plain, first, step;
+    plain.mode = EPI_PLAIN;
+    first.mode = EPI_FIRST;
+    first.alpha = 1.0 / ha;
+    first.beta = -c / ha;
+    step.mode = EPI_STEP;
+    step.alpha = 2.0 / ha;
+    step.beta = -2.0 * c / ha;
+    first.partials = step.partials = static_cast<double2 *>(ctx->partials.p);
+    int g = 0;
+    void *t0 = ctx->vec[0].p, *t1 = ctx->vec[1].p, *t2 = ctx->vec[2].p;
+    rc = stage_samples(ctx, vm, t0);
+    if (rc) return rc;
+    // Right: R~_n^a = T_n(A) B_a r; left: L~_n^a = B_a T_n(A) r
+    // (fftcheb_transport.hpp:290-430 with V_a = i B_a; the factors -i s of
+    // both sides multiply to s^2 in <L|R>)
+    for (int a = 0; a < 3 && !rc; ++a) {
+        rc = spmm(ctx, RG_SLOT_VX + a, vm, t0, Rst(a, 0), R, plain, nullptr);
+        if (!rc && M > 1) rc = apply_chain(ctx, vm, Rst(a, 0), Rst(a, 1), R, first, &g);
+        for (int n = 2; n < M && !rc; ++n) {
+            step.xp = Rst(a, n - 2);
+            rc = apply_chain(ctx, vm, Rst(a, n - 1), Rst(a, n), R, step, &g);
+            if (ctx->cancel.load()) {
+                (void)hipStreamSynchronize(ctx->stream);
+                drain_events(ctx);
+                return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+            }
+        }
+    }
+    if (rc) return rc;
+    for (int m = 0; m < M && !rc; ++m) {
+        void *bm = t0;
+        if (m == 1) {
+            rc = apply_chain(ctx, vm, t0, t1, R, first, &g);
+            bm = t1;
+        } else if (m >= 2) {
+            step.xp = t0;
+            rc = apply_chain(ctx, vm, t1, t2, R, step, &g);
+            std::swap(t0, t1); // t0 = T_{m-1} r
+            std::swap(t1, t2); // t1 = T_m r, t2 free
+            bm = t1;
+        }
+        for (int a = 0; a < 3 && !rc; ++a) rc = spmm(ctx, RG_SLOT_VX + a, vm, bm, Lst(a, m), R, plain, nullptr);
+        if (ctx->cancel.load()) {
+            (void)hipStreamSynchronize(ctx->stream);
+            drain_events(ctx);
+            return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+        }
+    }
+    if (rc) return rc;
+
+    // mu^a = s^2 L~^a^H R~^a, s = 1 / (2 a): one GEMM per axis, K = rows R
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+            return fail(ctx, RG_ERR_HIP, "rocblas_create_handle");
+    }
+    if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+        return fail(ctx, RG_ERR_HIP, "rocblas_set_stream");
+    const size_t mu_bytes = 3 * (size_t)M * M * 16;
+    RG_HIP(ctx, ctx->kg_mu.ensure(mu_bytes), "hipMalloc moment matrices");
+    const double s2 = 1.0 / (4.0 * ha * ha);
+    for (int a = 0; a < 3; ++a) {
+        rocblas_status st;
+        char *C = static_cast<char *>(ctx->kg_mu.p) + (size_t)a * M * M * (vm.complex_vec ? 16 : 8);
+        if (vm.complex_vec) {
+            const rocblas_double_complex alpha{s2, 0.0}, beta{0.0, 0.0};
+            st = rocblas_zgemm(ctx->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, M, M,
+                               (rocblas_int)K, &alpha, static_cast<const rocblas_double_complex *>(Lst(a, 0)),
+                               (rocblas_int)K, static_cast<const rocblas_double_complex *>(Rst(a, 0)),
+                               (rocblas_int)K, &beta, reinterpret_cast<rocblas_double_complex *>(C), M);
+        } else {
+            const double alpha = s2, beta = 0.0;
+            st = rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, M, M, (rocblas_int)K,
+                               &alpha, static_cast<const double *>(Lst(a, 0)), (rocblas_int)K,
+                               static_cast<const double *>(Rst(a, 0)), (rocblas_int)K, &beta,
+                               reinterpret_cast<double *>(C), M);
+        }
+        if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "moment GEMM failed");
+    }
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->prof) drain_events(ctx);
+    // host: mu in column-major C[m + n M] (complex or real)
+    std::vector<double> mu((size_t)3 * M * M * (vm.complex_vec ? 2 : 1));
+    RG_HIP(ctx, hipMemcpy(mu.data(), ctx->kg_mu.p, mu.size() * 8, hipMemcpyDeviceToHost), "download moments");
+    auto re_mu = [&](int a, int m, int n) -> double {
+        const size_t q = (size_t)a * M * M + (size_t)n * M + m;
+        return vm.complex_vec ? mu[2 * q] : mu[q];
+    };
+    if (mu_out) // [axis][m][n] complex, m the left index
+        for (int a = 0; a < 3; ++a)
+            for (int m = 0; m < M; ++m)
+                for (int n = 0; n < M; ++n) {
+                    const size_t q = (size_t)a * M * M + (size_t)n * M + m;
+                    const size_t o = ((size_t)a * M * M + (size_t)m * M + n) * 2;
+                    mu_out[o] = vm.complex_vec ? mu[2 * q] : mu[q];
+                    mu_out[o + 1] = vm.complex_vec ? mu[2 * q + 1] : 0.0;
+                }
+    // Greenwood on the Chebyshev nodes (fftcheb_transport.hpp:436-466, 570-623;
+    // the moment form of test_fftcheb_transport.cpp:239-262):
+    // kernel_k = sum_{m,n} w_m(k) w_n(k) Re mu_mn / max(1e-14, 1 - eps_k^2),
+    // w_m(k) = 2 g_m / (pi (1 + delta_m0)) T_m(eps_k)
+    std::vector<double> W((size_t)M * M), P((size_t)M);
+    for (int k = 0; k < M; ++k) {
+        const double th = M_PI * (k + 0.5) / M;
+        for (int m = 0; m < M; ++m) W[(size_t)k * M + m] = 2.0 * gk[m] / (M_PI * (m == 0 ? 2.0 : 1.0)) * std::cos(m * th);
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < M; ++k) {
+            const double th = M_PI * (k + 0.5) / M;
+            const double eps = std::cos(th);
+            const double *w = &W[(size_t)k * M];
+            double acc = 0.0;
+            for (int m = 0; m < M; ++m) {
+                double pm = 0.0;
+                for (int n = 0; n < M; ++n) pm += re_mu(a, m, n) * w[n];
+                acc += w[m] * pm;
+            }
+            const double native = acc / std::max(1e-14, 1.0 - eps * eps);
+            sigma_out[(size_t)(M - 1 - k) * 3 + a] = prefactor * native / R;
+            if (energy_out && a == 0) energy_out[M - 1 - k] = c + ha * eps;
+        }
     return RG_OK;
 }
 
@@ -1599,26 +1838,8 @@ int rg_kpm_density(const double *mu, int M, const double *x, int64_t nx, double 
         for (int64_t j = 0; j < nx; ++j) rho[j] = 0.0;
         return RG_OK;
     }
-    std::vector<double> gk((size_t)M, 1.0);
-    const int n0 = M / 2;
-    switch (kernel) {
-    case RG_KERNEL_NONE: break;
-    case RG_KERNEL_JACKSON: {
-        const double denom = (double)(M + 1);
-        const double theta = M_PI / denom;
-        const double cot_theta = std::cos(theta) / std::sin(theta);
-        for (int i = 1; i < M; ++i)
-            gk[i] = ((denom - i) * std::cos(theta * i) + cot_theta * std::sin(theta * i)) / denom;
-        break;
-    }
-    case RG_KERNEL_COSTAPER:
-        for (int i = n0; i < M; ++i) gk[i] = 0.5 * (1.0 + std::cos(M_PI * (double)(i - n0) / (double)(M - n0)));
-        break;
-    case RG_KERNEL_VPTAPER:
-        for (int i = n0; i < M; ++i) gk[i] = (double)(M - i) / (double)(M - n0);
-        break;
-    default: return RG_ERR_INVALID;
-    }
+    std::vector<double> gk;
+    if (kpm_weights(kernel, M, gk) != RG_OK) return RG_ERR_INVALID;
     for (int64_t j = 0; j < nx; ++j) {
         const double xv = std::max(-1.0 + edge_inset, std::min(1.0 - edge_inset, x[j]));
         double t_prev = 1.0, t_curr = xv;

@@ -162,11 +162,12 @@ hipError_t launch_sk_edge_blocks(const SkDev &t, int64_t n_edges, const int32_t 
                                  const SkCell &cell, const int32_t *edge_index, const int32_t *edge_shift,
                                  double *eblk, uint8_t *flags, hipStream_t st);
 // stored block s = sum over q in [src_ptr[s], src_ptr[s+1]) of source src[q]
-// (-1: the onsite diagonal of row_of[s]'s atom, else edge src[q]) times
-// phase[q] when cplx; written column-major (interleaved re, im when cplx)
+// (-1: the onsite diagonal of row_of[s]'s atom, else edge src[q]); mode 0
+// plain sum, 1 times the complex phase[q] (output interleaved re, im), 2
+// times the real weight phase[q].x; written column-major
 hipError_t launch_sk_gather(int64_t nnzb, int b, const int64_t *src_ptr, const int32_t *src,
                             const double2 *phase, const int32_t *row_of, const int32_t *type,
-                            const double *onsite, const double *eblk, double *vals, bool cplx,
+                            const double *onsite, const double *eblk, double *vals, int mode,
                             hipStream_t st);
 
 } // namespace rg

@@ -13,8 +13,8 @@
 // then, once the host has laid out the pattern of the kept blocks,
 //   k_sk_gather       sums each stored block's sources in a fixed order (the
 //                     onsite diagonal, then its edges in ascending order,
-//                     times their k phases) into the column-major device
-//                     block layout.
+//                     times their k phases, or times a bond component for the
+//                     velocity factors) into the column-major device layout.
 // All arithmetic is FP64; this is setup work, bound by the edge count.
 #include "rg_internal.h"
 
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_sk_gather(int64_t nnzb, int b, const in
                                                    const int32_t *__restrict__ src, const double2 *__restrict__ phase,
                                                    const int32_t *__restrict__ row_of, const int32_t *__restrict__ type,
                                                    const double *__restrict__ onsite, const double *__restrict__ eblk,
-                                                   double *__restrict__ vals, int cplx) {
+                                                   double *__restrict__ vals, int mode) {
     const int bb = b * b;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nnzb * bb) return;
@@ -242,17 +242,19 @@ __global__ __launch_bounds__(256) void k_sk_gather(int64_t nnzb, int b, const in
             x = i == j ? onsite[type[row_of[s]] * b + i] : 0.0;
         else
             x = eblk[(int64_t)e * bb + el];
-        if (cplx) {
+        if (mode == 1) { // k phase
             const double2 ph = phase[q];
             re += x * ph.x;
             im += x * ph.y;
+        } else if (mode == 2) { // real weight (a bond component)
+            re += x * phase[q].x;
         } else {
             re += x;
         }
     }
     // column-major block: element (i, j) at j b + i
     const int64_t o = s * bb + (int64_t)j * b + i;
-    if (cplx) {
+    if (mode == 1) {
         vals[2 * o] = re;
         vals[2 * o + 1] = im;
     } else {
@@ -284,11 +286,11 @@ int sk_max_pair_channels() { return kSkMaxPairCh; }
 
 hipError_t launch_sk_gather(int64_t nnzb, int b, const int64_t *src_ptr, const int32_t *src, const double2 *phase,
                             const int32_t *row_of, const int32_t *type, const double *onsite, const double *eblk,
-                            double *vals, bool cplx, hipStream_t st) {
+                            double *vals, int mode, hipStream_t st) {
     const int64_t n = nnzb * (int64_t)b * b;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_sk_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nnzb, b, src_ptr, src,
-                       phase, row_of, type, onsite, eblk, vals, cplx ? 1 : 0);
+                       phase, row_of, type, onsite, eblk, vals, mode);
     return hipGetLastError();
 }
 

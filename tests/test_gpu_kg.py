@@ -1,0 +1,85 @@
+"""Device Kubo-Greenwood (rg_sk_velocity + rg_kubo_greenwood) against the
+oracle: velocity factors, the sample-summed moment matrices mu = <L_m|R_n>
+and the Greenwood spectrum (both the moment form and the reference's
+FFTCheb route), to 1e-10 relative (FP64; the device contracts mu with a
+GEMM in another summation order)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import kg_oracle as kg
+from oracle import sk_oracle as so
+from rsatb_amd import _lib
+from rsatb_amd.bsr import BSR
+from sk_cases import periodic_edges, silicon_cell, sp3_silicon_table
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = _lib.Context(0)
+    yield c
+    c.close()
+
+
+def setup_si(ctx):
+    t = sp3_silicon_table()
+    pos, cell = silicon_cell()
+    ei, sh = periodic_edges(pos, cell, 3.2)
+    types = np.zeros(len(pos), dtype=np.int32)
+    a = t.arrays("H")
+    ctx.sk_assemble(_lib.SLOT_H, a, types, pos, cell, ei.T.copy(), sh)
+    ctx.sk_velocity(a, types, pos, cell, ei.T.copy(), sh)
+    rp, ci, blk = so.assemble(t, "H", types, pos, cell, ei, sh)
+    H = BSR(len(pos), 9, rp, ci, blk).to_dense()
+    vecs = so.edge_vectors(pos, cell, ei, sh)
+    blocks = [so.pair_block(t, "H", 0, 0, v) for v in vecs]
+    blocks = [np.zeros((9, 9)) if bk is None else bk for bk in blocks]
+    V = kg.bond_velocity(len(pos), 9, ei[0], ei[1], vecs, blocks)
+    return H, V
+
+
+def test_velocity_factors(ctx):
+    H, V = setup_si(ctx)
+    for a, slot in enumerate((_lib.SLOT_VX, _lib.SLOT_VY, _lib.SLOT_VZ)):
+        B = ctx.download_bsr(slot).to_dense()
+        assert np.max(np.abs(1j * B - V[a])) < 1e-12
+
+
+@pytest.mark.parametrize("kind", [_lib.SAMPLES_RADEMACHER, _lib.SAMPLES_PHASE])
+def test_greenwood_against_oracle(ctx, kind):
+    H, V = setup_si(ctx)
+    e = np.linalg.eigvalsh(H)
+    c, half = 0.5 * (e[0] + e[-1]), 0.55 * (e[-1] - e[0])
+    ctx.set_operator(_lib.FORM_ORTHOGONAL, c, half)
+    R, M = 4, 48
+    ctx.set_samples(H.shape[0], R, kind, 11)
+    print("case: Si spd cell KG, samples", kind, flush=True)
+    sigma, energy, mu = ctx.kubo_greenwood(M, prefactor=2.5, with_moments=True)
+    psi = ctx.get_samples()
+    L, Rs = kg.sequences(H, V, psi, c, half, M)
+    ref_mu = kg.moment_matrix(L, Rs, M)
+    assert np.max(np.abs(mu - ref_mu)) < 1e-10 * np.max(np.abs(ref_mu))
+    ref = kg.greenwood_from_moments(ref_mu, M, n_samples=R, prefactor=2.5)
+    fft = kg.greenwood_fftcheb(L, Rs, M, n_samples=R, prefactor=2.5)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(sigma - ref)) < 1e-10 * scale
+    assert np.max(np.abs(sigma - fft)) < 1e-10 * scale
+    assert np.allclose(energy, kg.energies(M, c, half), atol=1e-12)
+
+
+def test_greenwood_needs_velocity_and_orthogonal_form(ctx):
+    c2 = _lib.Context(0)
+    try:
+        t = sp3_silicon_table()
+        pos, cell = silicon_cell()
+        ei, sh = periodic_edges(pos, cell, 3.2)
+        c2.sk_assemble(_lib.SLOT_H, t.arrays("H"), np.zeros(len(pos), np.int32), pos, cell, ei.T.copy(), sh)
+        c2.set_operator(_lib.FORM_ORTHOGONAL, 0.0, 30.0)
+        c2.set_samples(72, 2, _lib.SAMPLES_RADEMACHER, 1)
+        with pytest.raises(_lib.RsatbGpuError, match="velocity"):
+            c2.kubo_greenwood(8)
+    finally:
+        c2.close()
