@@ -51,6 +51,11 @@ def parse():
     p.add_argument("--no-sk", action="store_true", help="skip the Slater-Koster assembly leg")
     p.add_argument("--sk-cells", type=int, default=16, help="diamond Si cells per edge (SK assembly leg)")
     p.add_argument("--sk-cpu-edges", type=int, default=20000, help="edge blocks the CPU oracle evaluates")
+    p.add_argument("--no-kg", action="store_true", help="skip the Kubo-Greenwood leg")
+    p.add_argument("--kg-cells", type=int, default=128, help="graphene rectangular cells per edge (KG leg)")
+    p.add_argument("--kg-samples", type=int, default=16)
+    p.add_argument("--kg-moments", type=int, default=512)
+    p.add_argument("--kg-cpu-moments", type=int, default=128)
     return p.parse_args()
 
 
@@ -250,6 +255,70 @@ def sk_leg(cells: int, reps: int, cpu_edges: int) -> dict:
             "gpu_over_cpu": round(gpu_rate / cpu_rate, 1)}
 
 
+def kg_leg(cells: int, R: int, M: int, cpu_M: int) -> dict:
+    """Kubo-Greenwood spectrum (rg_sk_velocity + rg_kubo_greenwood) on
+    graphene, cells x cells rectangular cells, one p_z orbital, nearest-
+    neighbour SK hopping; R Rademacher samples, M moments (Jackson). The CPU
+    leg runs the oracle (scipy sparse sequences, FP64) with cpu_M moments and
+    one sample."""
+    import numpy as np
+
+    from oracle import kg_oracle as kg
+    from rsatb_amd import _lib
+    from rsatb_amd.lattice import graphene_rect_supercell, neighbor_edges
+    from rsatb_amd.sktb import Radial, SkParameters, SktbModel
+
+    h = SkParameters().set_onsite("C", "pz", 0.0)
+    h.set_pair_channel("C", "C", "pp", "pi", Radial.spline([0.0, 1.6], [-2.7, -2.7], 1.6))
+    model = SktbModel({"C": ["pz"]}, h)
+    st = graphene_rect_supercell(cells, cells)
+    e = neighbor_edges(st, model.cutoff)
+    pairs = np.stack([e.src, e.dst], axis=1)
+    N = st.n_atoms
+    with _lib.Context(0) as c:
+        args = (st.atom_type, st.positions, st.cell, pairs, e.shift)
+        c.sk_assemble(_lib.SLOT_H, model.arrays("H"), *args)
+        c.sk_velocity(model.arrays("H"), *args)
+        c.set_operator(_lib.FORM_ORTHOGONAL, 0.0, 8.2)
+        c.set_samples(N, R, _lib.SAMPLES_RADEMACHER, 20240917)
+        c.kubo_greenwood(16)  # warm-up
+        c.set_profiling(True)
+        t0 = time.perf_counter()
+        sigma, energy = c.kubo_greenwood(M)
+        dt = time.perf_counter() - t0
+        ks = {name: c.kernel_stats(k) for k, name in _lib.KCLASS_NAMES.items()}
+        c.set_profiling(False)
+        # the same operator for the CPU leg (b = 1: one scalar per block)
+        Hb = c.download_bsr(_lib.SLOT_H)
+        Bb = [c.download_bsr(sl) for sl in (_lib.SLOT_VX, _lib.SLOT_VY, _lib.SLOT_VZ)]
+    gemm = ks["kg_gemm"]
+    g_tf = gemm["flops"] / (gemm["avg_ms"] * 1e-3) / 1e12 if gemm["count"] else 0.0
+    # CPU: the oracle's sparse sequences and moment matrix for one sample
+    import scipy.sparse as sp
+
+    def csr(b, scale=1.0):
+        return sp.csr_matrix((scale * b.values.reshape(-1), b.col_ind, b.row_ptr), shape=(N, N))
+
+    Hs = csr(Hb)
+    Vs = [csr(b, 1j) for b in Bb]
+    r = np.random.default_rng(1).choice([-1.0, 1.0], size=N) / np.sqrt(N)
+    t1 = time.perf_counter()
+    L, Rq = kg.sequences(Hs, Vs, r, 0.0, 8.2, cpu_M)
+    kg.greenwood_from_moments(kg.moment_matrix(L, Rq, cpu_M), cpu_M)
+    ct = time.perf_counter() - t1
+    units = float(N) * R * M
+    return {"workload": f"graphene {cells}x{cells} rectangular cells ({N} atoms, p_z), nearest-neighbour SK, "
+                        f"R={R} Rademacher samples, M={M} moments, Greenwood xx/yy/zz, FP64",
+            "ms": round(dt * 1e3, 2), "value": units / dt, "unit": "orbital*vector*moment/s",
+            "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "count": v["count"]} for k, v in ks.items()
+                        if v["count"]},
+            "gemm_TFLOPs": round(g_tf, 2), "gemm_frac_of_fp64_peak": round(g_tf / 78.6, 3),
+            "sigma_xx_max": float(np.max(sigma[:, 0])),
+            "cpu_baseline": {"value": float(N) * cpu_M / ct, "unit": "orbital*vector*moment/s", "cores": 1,
+                             "kind": "port",
+                             "sample": f"one sample, {cpu_M} moments, scipy sparse oracle, {ct:.2f} s"}}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -320,6 +389,8 @@ def main() -> int:
         line["adjoint"] = adjoint_leg(args.adj_cells, max(1, args.steps))
     if not args.no_sk:
         line["sk_assembly"] = sk_leg(args.sk_cells, 3, args.sk_cpu_edges)
+    if not args.no_kg:
+        line["kubo_greenwood"] = kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(calc, args.cpu_moments, mu)
     print(json.dumps(line), flush=True)

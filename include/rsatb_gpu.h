@@ -305,7 +305,8 @@ RG_API int rg_set_profiling(rg_context *ctx, int on);
 #define RG_KCLASS_SPMM_STEP 1  /* SpMM fused with the recurrence and dot partials */
 #define RG_KCLASS_REDUCE 2     /* dot-partial reduction */
 #define RG_KCLASS_GROUP 3      /* group-resolved dots of the projected moments */
-#define RG_KCLASS_COUNT 4
+#define RG_KCLASS_KG_GEMM 4    /* Kubo-Greenwood moment GEMM mu = L^H R (one axis) */
+#define RG_KCLASS_COUNT 5
 
 /* Average launch duration (ms), launch count, and the algorithmic bytes and
  * flops of one launch of that class as last launched. */

@@ -45,7 +45,14 @@ def nodes(M: int) -> np.ndarray:
 
 def sequences(H, V, r, center: float, half: float, M: int):
     """(L, R): [3][M] arrays of the sample shape (r: [rows] or [rows, S])."""
-    A = (np.asarray(H) - center * np.eye(H.shape[0])) / half
+    try:
+        import scipy.sparse as sp
+    except ImportError:  # dense only
+        sp = None
+    if sp is not None and sp.issparse(H):
+        A = (H - center * sp.identity(H.shape[0], format="csr")) / half
+    else:
+        A = (np.asarray(H) - center * np.eye(H.shape[0])) / half
     s = 1.0 / (2.0 * half)
     r = np.asarray(r, dtype=complex)
     L = [[None] * M for _ in range(3)]

@@ -29,9 +29,9 @@ SAMPLES_RADEMACHER, SAMPLES_PHASE, SAMPLES_UPLOAD_REAL, SAMPLES_UPLOAD_COMPLEX =
 PREC_FP64, PREC_FP32 = 0, 1
 DOUBLING_AUTO, DOUBLING_OFF, DOUBLING_ON = -1, 0, 1
 KERNEL_NONE, KERNEL_JACKSON, KERNEL_COSTAPER, KERNEL_VPTAPER = 0, 1, 2, 3
-KCLASS_SPMM_PLAIN, KCLASS_SPMM_STEP, KCLASS_REDUCE, KCLASS_GROUP = 0, 1, 2, 3
+KCLASS_SPMM_PLAIN, KCLASS_SPMM_STEP, KCLASS_REDUCE, KCLASS_GROUP, KCLASS_KG_GEMM = 0, 1, 2, 3, 4
 KCLASS_NAMES = {KCLASS_SPMM_PLAIN: "spmm_plain", KCLASS_SPMM_STEP: "spmm_step", KCLASS_REDUCE: "reduce",
-                KCLASS_GROUP: "group_dots"}
+                KCLASS_GROUP: "group_dots", KCLASS_KG_GEMM: "kg_gemm"}
 
 # Every symbol include/rsatb_gpu.h declares, with (restype, argtypes).
 _vp, _i, _i64, _d, _u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64

@@ -1338,10 +1338,20 @@ int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, doub
     if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
         return fail(ctx, RG_ERR_HIP, "rocblas_set_stream");
     const size_t mu_bytes = 3 * (size_t)M * M * 16;
-    RG_HIP(ctx, ctx->kg_mu.ensure(mu_bytes), "hipMalloc moment matrices");
+    const int ncomp = vm.complex_vec ? 2 : 1;
+    const size_t mat = (size_t)M * M * ncomp * 8; // one M x M node matrix
+    // mu of the three axes, then W and Q of each axis (sized before any write)
+    RG_HIP(ctx, ctx->kg_mu.ensure(mu_bytes + 4 * mat), "hipMalloc moment matrices");
     const double s2 = 1.0 / (4.0 * ha * ha);
     for (int a = 0; a < 3; ++a) {
         rocblas_status st;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (ctx->prof) {
+            ev0 = take_event(ctx);
+            ev1 = take_event(ctx);
+            if (!ev0 || !ev1) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
+            RG_HIP(ctx, hipEventRecord(ev0, ctx->stream), "hipEventRecord");
+        }
         char *C = static_cast<char *>(ctx->kg_mu.p) + (size_t)a * M * M * (vm.complex_vec ? 16 : 8);
         if (vm.complex_vec) {
             const rocblas_double_complex alpha{s2, 0.0}, beta{0.0, 0.0};
@@ -1357,45 +1367,71 @@ int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, doub
                                reinterpret_cast<double *>(C), M);
         }
         if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "moment GEMM failed");
+        if (ctx->prof) {
+            RG_HIP(ctx, hipEventRecord(ev1, ctx->stream), "hipEventRecord");
+            ctx->pending.push_back({ev0, ev1, RG_KCLASS_KG_GEMM});
+            // algorithmic: both sequence blocks read once, mu written once
+            const double cplx = vm.complex_vec ? 2.0 : 1.0;
+            ctx->stats[RG_KCLASS_KG_GEMM].bytes = 2.0 * (double)M * (double)K * eb + (double)M * M * 8 * cplx;
+            ctx->stats[RG_KCLASS_KG_GEMM].flops = 2.0 * (double)M * M * (double)K * (vm.complex_vec ? 4.0 : 1.0);
+        }
+    }
+    // Greenwood on the Chebyshev nodes (fftcheb_transport.hpp:436-466, 570-623;
+    // the moment form of test_fftcheb_transport.cpp:239-262):
+    // kernel_k = sum_{m,n} W[m,k] W[n,k] Re mu_mn / max(1e-14, 1 - eps_k^2),
+    // W[m,k] = 2 g_m / (pi (1 + delta_m0)) T_m(eps_k). Q = mu W on the device
+    // (W real, so Re Q = Re(mu) W), then the diagonal sum on the host.
+    std::vector<double> W((size_t)M * M * ncomp, 0.0); // column-major W[m + k M]
+    for (int k = 0; k < M; ++k) {
+        const double th = M_PI * (k + 0.5) / M;
+        for (int m = 0; m < M; ++m)
+            W[((size_t)k * M + m) * ncomp] = 2.0 * gk[m] / (M_PI * (m == 0 ? 2.0 : 1.0)) * std::cos(m * th);
+    }
+    char *Wd = static_cast<char *>(ctx->kg_mu.p) + mu_bytes, *Qd = Wd + mat;
+    RG_HIP(ctx, hipMemcpyAsync(Wd, W.data(), mat, hipMemcpyHostToDevice, ctx->stream), "upload node weights");
+    for (int a = 0; a < 3; ++a) {
+        rocblas_status st;
+        char *C = static_cast<char *>(ctx->kg_mu.p) + (size_t)a * M * M * (vm.complex_vec ? 16 : 8);
+        char *Q = Qd + (size_t)a * mat;
+        if (vm.complex_vec) {
+            const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+            st = rocblas_zgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
+                               reinterpret_cast<const rocblas_double_complex *>(C), M,
+                               reinterpret_cast<const rocblas_double_complex *>(Wd), M, &zero,
+                               reinterpret_cast<rocblas_double_complex *>(Q), M);
+        } else {
+            const double one = 1.0, zero = 0.0;
+            st = rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
+                               reinterpret_cast<const double *>(C), M, reinterpret_cast<const double *>(Wd), M,
+                               &zero, reinterpret_cast<double *>(Q), M);
+        }
+        if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "node GEMM failed");
+    }
+    std::vector<double> Qh((size_t)3 * M * M * ncomp);
+    RG_HIP(ctx, hipMemcpyAsync(Qh.data(), Qd, Qh.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+    std::vector<double> mu;
+    if (mu_out) {
+        mu.resize((size_t)3 * M * M * ncomp);
+        RG_HIP(ctx, hipMemcpyAsync(mu.data(), ctx->kg_mu.p, mu.size() * 8, hipMemcpyDeviceToHost, ctx->stream),
+               "download moments");
     }
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (ctx->prof) drain_events(ctx);
-    // host: mu in column-major C[m + n M] (complex or real)
-    std::vector<double> mu((size_t)3 * M * M * (vm.complex_vec ? 2 : 1));
-    RG_HIP(ctx, hipMemcpy(mu.data(), ctx->kg_mu.p, mu.size() * 8, hipMemcpyDeviceToHost), "download moments");
-    auto re_mu = [&](int a, int m, int n) -> double {
-        const size_t q = (size_t)a * M * M + (size_t)n * M + m;
-        return vm.complex_vec ? mu[2 * q] : mu[q];
-    };
     if (mu_out) // [axis][m][n] complex, m the left index
         for (int a = 0; a < 3; ++a)
             for (int m = 0; m < M; ++m)
                 for (int n = 0; n < M; ++n) {
                     const size_t q = (size_t)a * M * M + (size_t)n * M + m;
                     const size_t o = ((size_t)a * M * M + (size_t)m * M + n) * 2;
-                    mu_out[o] = vm.complex_vec ? mu[2 * q] : mu[q];
-                    mu_out[o + 1] = vm.complex_vec ? mu[2 * q + 1] : 0.0;
+                    mu_out[o] = mu[q * ncomp];
+                    mu_out[o + 1] = ncomp == 2 ? mu[q * ncomp + 1] : 0.0;
                 }
-    // Greenwood on the Chebyshev nodes (fftcheb_transport.hpp:436-466, 570-623;
-    // the moment form of test_fftcheb_transport.cpp:239-262):
-    // kernel_k = sum_{m,n} w_m(k) w_n(k) Re mu_mn / max(1e-14, 1 - eps_k^2),
-    // w_m(k) = 2 g_m / (pi (1 + delta_m0)) T_m(eps_k)
-    std::vector<double> W((size_t)M * M), P((size_t)M);
-    for (int k = 0; k < M; ++k) {
-        const double th = M_PI * (k + 0.5) / M;
-        for (int m = 0; m < M; ++m) W[(size_t)k * M + m] = 2.0 * gk[m] / (M_PI * (m == 0 ? 2.0 : 1.0)) * std::cos(m * th);
-    }
     for (int a = 0; a < 3; ++a)
         for (int k = 0; k < M; ++k) {
-            const double th = M_PI * (k + 0.5) / M;
-            const double eps = std::cos(th);
-            const double *w = &W[(size_t)k * M];
+            const double eps = std::cos(M_PI * (k + 0.5) / M);
             double acc = 0.0;
-            for (int m = 0; m < M; ++m) {
-                double pm = 0.0;
-                for (int n = 0; n < M; ++n) pm += re_mu(a, m, n) * w[n];
-                acc += w[m] * pm;
-            }
+            for (int m = 0; m < M; ++m)
+                acc += W[((size_t)k * M + m) * ncomp] * Qh[((size_t)a * M * M + (size_t)k * M + m) * ncomp];
             const double native = acc / std::max(1e-14, 1.0 - eps * eps);
             sigma_out[(size_t)(M - 1 - k) * 3 + a] = prefactor * native / R;
             if (energy_out && a == 0) energy_out[M - 1 - k] = c + ha * eps;

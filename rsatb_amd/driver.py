@@ -291,6 +291,59 @@ class RSATB:
             "population": "lowdin", "group_type": groups.group_type, "group_slot": groups.group_slot,
         }
 
+    def transport_volume(self, axis: str = "abc") -> float:
+        """TransportNormalizationVolume (rsatb/calc/observables.hpp:277-285):
+        a cell-vector length for one axis, the spanned area for two, the cell
+        volume for "abc"."""
+        cell = self.structure.cell
+        rows = ["abc".index(ch) for ch in axis] if axis and all(ch in "abc" for ch in axis) else None
+        if rows is None or len(set(rows)) != len(rows):
+            raise RuntimeError(f"unknown normalization axis '{axis}'")
+        if len(rows) == 1:
+            return float(np.linalg.norm(cell[rows[0]]))
+        if len(rows) == 2:
+            return float(np.linalg.norm(np.cross(cell[rows[0]], cell[rows[1]])))
+        return float(abs(np.linalg.det(cell)))
+
+    def compute_dc_spectrum(self, n_moments: int, formulation: str = "greenwood", axis: str = "abc",
+                            partition_sections: int = 0, max_workspace_mb: int = 0, kernel: str = "jackson",
+                            method: str = "fftcheb"):
+        """compute_dc_spectrum (rsatb/driver.py:366-396, observables.hpp:691-779):
+        the zero-temperature Greenwood conductivity on the Chebyshev nodes,
+        sigma (n_k, n_moments, 6) in S/m with both spins, components (xx, xy,
+        xz, yy, yz, zz) of which the Greenwood formulation fills the diagonal,
+        and the energies in eV. The left and right sequences are kept whole on
+        the device (partition_sections and max_workspace_mb are accepted and
+        do not apply)."""
+        if formulation != "greenwood":
+            raise RuntimeError("dc formulation must be 'greenwood' on the GPU path (Bastin is not provided)")
+        if method not in ("fftcheb", "qe"):
+            raise RuntimeError(f"dc method must be 'fftcheb' or 'qe', got '{method}'")
+        kernels = {"jackson": _lib.KERNEL_JACKSON, "costaper": _lib.KERNEL_COSTAPER, "none": _lib.KERNEL_NONE}
+        if kernel not in kernels:
+            raise RuntimeError(f"unknown kpm kernel '{kernel}'")
+        if not getattr(self.model, "device_assembly", False) or self.model.has_overlap():
+            raise RuntimeError("compute_dc_spectrum requires a velocity-capable model/operator setup: an "
+                               "orthogonal SktbModel (the overlap's composite velocity is not provided)")
+        if n_moments < 1:
+            raise RuntimeError("n_moments must be positive")
+        emin, emax = self.energy_bounds()
+        st, e = self.structure, self.edges
+        n_rows = st.n_atoms * self.model.norb
+        prefactor = np.pi * 2.0 * 4.0 * n_rows / self.transport_volume(axis)
+        sigma = np.zeros((len(self._kstates), n_moments, 6))
+        energy = None
+        for k_idx, ks in enumerate(self._kstates):
+            if np.any(ks["k"]):
+                raise RuntimeError("compute_dc_spectrum: the velocity is built at Gamma only")
+            self._bind_kpoint(k_idx)
+            self._ctx.sk_velocity(self.model.arrays("H"), st.atom_type, st.positions, st.cell,
+                                  np.stack([e.src, e.dst], axis=1), e.shift)
+            s3, energy = self._ctx.kubo_greenwood(n_moments, kernels[kernel], prefactor)
+            sigma[k_idx][:, [0, 3, 5]] = s3
+        # e^2 / (hbar Angstrom) in S/m (rsatb/driver.py:393)
+        return sigma * 2.4341348 * 1e6, energy
+
     def dump_operator_coo(self, which: str, axis: int = 0, k_idx: int = 0):
         """Global COO triplets of H or S (rsatb/pybind_rsatb.cpp:612-670)."""
         ks = self._kstates[k_idx]
