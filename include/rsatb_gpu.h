@@ -267,6 +267,15 @@ RG_API int rg_sk_velocity(rg_context *ctx, const rg_sk_table *table, int64_t n_a
 RG_API int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, double *sigma_out,
                              double *energy_out, double *mu_out);
 
+/* The full DC spectrum of dc_spectrum_fftcheb (fftcheb_transport.hpp:543-653)
+ * for the same inputs: formulation 0 Greenwood (the diagonal components) or
+ * 1 Bastin (all six, integrated over the energy: sigma at T = 0 as a function
+ * of the Fermi energy); sigma_out[k * 6 + comp], comp in (xx, xy, xz, yy, yz,
+ * zz); mu_out (may be NULL): [pairs][M][M] complex, 3 pairs (xx, yy, zz) for
+ * Greenwood, 6 in component order for Bastin. */
+RG_API int rg_dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor,
+                          double *sigma_out, double *energy_out, double *mu_out);
+
 /* ---- graph-local S^{-1/2} (graph_inverse_sqrt seed, rsatb/calc/isqrtm.hpp:137-232;
  *      ISQRTMPreconditioner ctor, rsatb/calc/preconditioner.hpp:184-192) ---- */
 

@@ -75,6 +75,7 @@ SIGNATURES = {
                             _pi64]),
     "rg_sk_velocity": (_i, [_vp, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32]),
     "rg_kubo_greenwood": (_i, [_vp, _i, _i, _d, _pd, _pd, _pd]),
+    "rg_dc_spectrum": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
     "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_set_checkpointing": (_i, [_vp, _i]),
@@ -238,6 +239,19 @@ class Context:
         rc = self._lib.rg_kubo_greenwood(self._h, M, int(kernel), float(prefactor), _dptr(sigma), _dptr(energy),
                                          _dptr(mu.view(np.float64)) if with_moments else None)
         _check(self._h, rc, "rg_kubo_greenwood")
+        return (sigma, energy, mu) if with_moments else (sigma, energy)
+
+    def dc_spectrum(self, n_moments: int, formulation: int = 0, kernel: int = KERNEL_JACKSON,
+                    prefactor: float = 1.0, with_moments: bool = False):
+        """(sigma [M, 6] in (xx, xy, xz, yy, yz, zz), energies [M] [, mu
+        [pairs, M, M]]); formulation 0 Greenwood, 1 Bastin."""
+        M = int(n_moments)
+        sigma = np.zeros((M, 6))
+        energy = np.zeros(M)
+        mu = np.zeros((6 if formulation == 1 else 3, M, M), dtype=np.complex128) if with_moments else None
+        rc = self._lib.rg_dc_spectrum(self._h, M, int(formulation), int(kernel), float(prefactor), _dptr(sigma),
+                                      _dptr(energy), _dptr(mu.view(np.float64)) if with_moments else None)
+        _check(self._h, rc, "rg_dc_spectrum")
         return (sigma, energy, mu) if with_moments else (sigma, energy)
 
     def _sk_call(self, velocity: bool, slot: int, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift,

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <complex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1243,9 +1244,19 @@ int rg_sk_velocity(rg_context *ctx, const rg_sk_table *t, int64_t n_atoms, const
                     true, nullptr);
 }
 
-int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, double *sigma_out,
-                      double *energy_out, double *mu_out) {
+} // extern "C"
+
+namespace {
+
+// DC conductivity spectrum (dc_spectrum_fftcheb, rsatb/calc/fftcheb_transport.hpp:
+// 543-653) of the orthogonal form: sigma_out[k * 6 + comp], comp in (xx, xy,
+// xz, yy, yz, zz). Greenwood (formulation 0) fills the diagonal from the
+// diagonal pairs; Bastin (1) all six from mu^{ab}, a <= b. mu_out: [pairs][M][M]
+// complex, pairs 3 (Greenwood: xx, yy, zz) or 6 (Bastin, component order).
+int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor, double *sigma_out,
+                double *energy_out, double *mu_out) {
     if (!ctx || !sigma_out) return RG_ERR_INVALID;
+    if (formulation != 0 && formulation != 1) return fail(ctx, RG_ERR_INVALID, "formulation: 0 Greenwood, 1 Bastin");
     int rc = check_operator(ctx);
     if (rc) return rc;
     if (ctx->form != RG_FORM_ORTHOGONAL)
@@ -1330,20 +1341,32 @@ int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, doub
     }
     if (rc) return rc;
 
-    // mu^a = s^2 L~^a^H R~^a, s = 1 / (2 a): one GEMM per axis, K = rows R
+    // mu^{ab} = s^2 L~^a^H R~^b, s = 1 / (2 a): one GEMM per axis pair, K = rows R
     if (!ctx->blas) {
         if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
             return fail(ctx, RG_ERR_HIP, "rocblas_create_handle");
     }
     if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
         return fail(ctx, RG_ERR_HIP, "rocblas_set_stream");
-    const size_t mu_bytes = 3 * (size_t)M * M * 16;
-    const int ncomp = vm.complex_vec ? 2 : 1;
-    const size_t mat = (size_t)M * M * ncomp * 8; // one M x M node matrix
-    // mu of the three axes, then W and Q of each axis (sized before any write)
-    RG_HIP(ctx, ctx->kg_mu.ensure(mu_bytes + 4 * mat), "hipMalloc moment matrices");
+    const int comp_a[6] = {0, 0, 0, 1, 1, 2}, comp_b[6] = {0, 1, 2, 1, 2, 2};
+    std::vector<int> pairs;
+    for (int q = 0; q < 6; ++q)
+        if (formulation == 1 || comp_a[q] == comp_b[q]) pairs.push_back(q);
+    const int np = (int)pairs.size();
+    const size_t mm = (size_t)M * M;
+    const size_t mu_real = (size_t)np * mm * (vm.complex_vec ? 2 : 1); // doubles
+    // device layout: mu as computed, then complex mu (Bastin or complex
+    // samples), three complex weight matrices, three complex Q per pair
+    const size_t zmat = mm * 2;
+    RG_HIP(ctx, ctx->kg_mu.ensure((mu_real + (size_t)np * zmat + 3 * zmat + 3 * (size_t)np * zmat) * 8),
+           "hipMalloc moment matrices");
+    double *mud = static_cast<double *>(ctx->kg_mu.p);
+    double *muz = mud + mu_real;
+    double *Wz = muz + (size_t)np * zmat;
+    double *Qz = Wz + 3 * zmat;
     const double s2 = 1.0 / (4.0 * ha * ha);
-    for (int a = 0; a < 3; ++a) {
+    for (int pi = 0; pi < np; ++pi) {
+        const int a = comp_a[pairs[pi]], b = comp_b[pairs[pi]];
         rocblas_status st;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (ctx->prof) {
@@ -1352,19 +1375,18 @@ int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, doub
             if (!ev0 || !ev1) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
             RG_HIP(ctx, hipEventRecord(ev0, ctx->stream), "hipEventRecord");
         }
-        char *C = static_cast<char *>(ctx->kg_mu.p) + (size_t)a * M * M * (vm.complex_vec ? 16 : 8);
         if (vm.complex_vec) {
             const rocblas_double_complex alpha{s2, 0.0}, beta{0.0, 0.0};
             st = rocblas_zgemm(ctx->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, M, M,
                                (rocblas_int)K, &alpha, static_cast<const rocblas_double_complex *>(Lst(a, 0)),
-                               (rocblas_int)K, static_cast<const rocblas_double_complex *>(Rst(a, 0)),
-                               (rocblas_int)K, &beta, reinterpret_cast<rocblas_double_complex *>(C), M);
+                               (rocblas_int)K, static_cast<const rocblas_double_complex *>(Rst(b, 0)),
+                               (rocblas_int)K, &beta, reinterpret_cast<rocblas_double_complex *>(mud + pi * zmat),
+                               M);
         } else {
             const double alpha = s2, beta = 0.0;
             st = rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, M, M, (rocblas_int)K,
                                &alpha, static_cast<const double *>(Lst(a, 0)), (rocblas_int)K,
-                               static_cast<const double *>(Rst(a, 0)), (rocblas_int)K, &beta,
-                               reinterpret_cast<double *>(C), M);
+                               static_cast<const double *>(Rst(b, 0)), (rocblas_int)K, &beta, mud + pi * mm, M);
         }
         if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "moment GEMM failed");
         if (ctx->prof) {
@@ -1372,70 +1394,141 @@ int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, doub
             ctx->pending.push_back({ev0, ev1, RG_KCLASS_KG_GEMM});
             // algorithmic: both sequence blocks read once, mu written once
             const double cplx = vm.complex_vec ? 2.0 : 1.0;
-            ctx->stats[RG_KCLASS_KG_GEMM].bytes = 2.0 * (double)M * (double)K * eb + (double)M * M * 8 * cplx;
-            ctx->stats[RG_KCLASS_KG_GEMM].flops = 2.0 * (double)M * M * (double)K * (vm.complex_vec ? 4.0 : 1.0);
+            ctx->stats[RG_KCLASS_KG_GEMM].bytes = 2.0 * (double)M * (double)K * eb + (double)mm * 8 * cplx;
+            ctx->stats[RG_KCLASS_KG_GEMM].flops = 2.0 * (double)mm * (double)K * (vm.complex_vec ? 4.0 : 1.0);
         }
     }
-    // Greenwood on the Chebyshev nodes (fftcheb_transport.hpp:436-466, 570-623;
-    // the moment form of test_fftcheb_transport.cpp:239-262):
-    // kernel_k = sum_{m,n} W[m,k] W[n,k] Re mu_mn / max(1e-14, 1 - eps_k^2),
-    // W[m,k] = 2 g_m / (pi (1 + delta_m0)) T_m(eps_k). Q = mu W on the device
-    // (W real, so Re Q = Re(mu) W), then the diagonal sum on the host.
-    std::vector<double> W((size_t)M * M * ncomp, 0.0); // column-major W[m + k M]
+    // node weights (complex, column-major [n + k M]), theta_k = pi (k + 1/2) / M:
+    //   GC[n,k] = g_n cos(n theta_k), GE[n,k] = g_n e^{i n theta_k}, GNE = n GE,
+    //   g_n = 2 j_n / (pi (1 + delta_n0))    (fftcheb_transport.hpp:259-273)
+    std::vector<double> gw((size_t)M), Wh(3 * zmat, 0.0);
+    for (int n = 0; n < M; ++n) gw[n] = 2.0 * gk[n] / (M_PI * (n == 0 ? 2.0 : 1.0));
     for (int k = 0; k < M; ++k) {
         const double th = M_PI * (k + 0.5) / M;
-        for (int m = 0; m < M; ++m)
-            W[((size_t)k * M + m) * ncomp] = 2.0 * gk[m] / (M_PI * (m == 0 ? 2.0 : 1.0)) * std::cos(m * th);
-    }
-    char *Wd = static_cast<char *>(ctx->kg_mu.p) + mu_bytes, *Qd = Wd + mat;
-    RG_HIP(ctx, hipMemcpyAsync(Wd, W.data(), mat, hipMemcpyHostToDevice, ctx->stream), "upload node weights");
-    for (int a = 0; a < 3; ++a) {
-        rocblas_status st;
-        char *C = static_cast<char *>(ctx->kg_mu.p) + (size_t)a * M * M * (vm.complex_vec ? 16 : 8);
-        char *Q = Qd + (size_t)a * mat;
-        if (vm.complex_vec) {
-            const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
-            st = rocblas_zgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
-                               reinterpret_cast<const rocblas_double_complex *>(C), M,
-                               reinterpret_cast<const rocblas_double_complex *>(Wd), M, &zero,
-                               reinterpret_cast<rocblas_double_complex *>(Q), M);
-        } else {
-            const double one = 1.0, zero = 0.0;
-            st = rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
-                               reinterpret_cast<const double *>(C), M, reinterpret_cast<const double *>(Wd), M,
-                               &zero, reinterpret_cast<double *>(Q), M);
+        for (int n = 0; n < M; ++n) {
+            const size_t o = 2 * ((size_t)k * M + n);
+            Wh[o] = gw[n] * std::cos(n * th);
+            Wh[zmat + o] = gw[n] * std::cos(n * th);
+            Wh[zmat + o + 1] = gw[n] * std::sin(n * th);
+            Wh[2 * zmat + o] = n * gw[n] * std::cos(n * th);
+            Wh[2 * zmat + o + 1] = n * gw[n] * std::sin(n * th);
         }
-        if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "node GEMM failed");
     }
-    std::vector<double> Qh((size_t)3 * M * M * ncomp);
-    RG_HIP(ctx, hipMemcpyAsync(Qh.data(), Qd, Qh.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
-    std::vector<double> mu;
-    if (mu_out) {
-        mu.resize((size_t)3 * M * M * ncomp);
-        RG_HIP(ctx, hipMemcpyAsync(mu.data(), ctx->kg_mu.p, mu.size() * 8, hipMemcpyDeviceToHost, ctx->stream),
-               "download moments");
+    RG_HIP(ctx, hipMemcpyAsync(Wz, Wh.data(), Wh.size() * 8, hipMemcpyHostToDevice, ctx->stream),
+           "upload node weights");
+    // complex copies of mu (a real mu widened on the host)
+    std::vector<double> mu_h(mu_real);
+    RG_HIP(ctx, hipMemcpyAsync(mu_h.data(), mud, mu_real * 8, hipMemcpyDeviceToHost, ctx->stream),
+           "download moments");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    std::vector<double> muz_h((size_t)np * zmat);
+    for (size_t q = 0; q < (size_t)np * mm; ++q) {
+        muz_h[2 * q] = vm.complex_vec ? mu_h[2 * q] : mu_h[q];
+        muz_h[2 * q + 1] = vm.complex_vec ? mu_h[2 * q + 1] : 0.0;
     }
+    RG_HIP(ctx, hipMemcpyAsync(muz, muz_h.data(), muz_h.size() * 8, hipMemcpyHostToDevice, ctx->stream),
+           "upload moments");
+    // Q_t = mu W_t (Greenwood: t = GC only; Bastin: GE, GC, GNE)
+    const int nq = formulation == 1 ? 3 : 1;
+    for (int pi = 0; pi < np; ++pi)
+        for (int t = 0; t < nq; ++t) {
+            const int wsel = formulation == 1 ? (t == 0 ? 1 : (t == 1 ? 0 : 2)) : 0;
+            const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+            const rocblas_status st = rocblas_zgemm(
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
+                reinterpret_cast<const rocblas_double_complex *>(muz + pi * zmat), M,
+                reinterpret_cast<const rocblas_double_complex *>(Wz + wsel * zmat), M, &zero,
+                reinterpret_cast<rocblas_double_complex *>(Qz + ((size_t)pi * 3 + t) * zmat), M);
+            if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "node GEMM failed");
+        }
+    std::vector<double> Qh((size_t)np * 3 * zmat);
+    RG_HIP(ctx, hipMemcpyAsync(Qh.data(), Qz, Qh.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (ctx->prof) drain_events(ctx);
-    if (mu_out) // [axis][m][n] complex, m the left index
-        for (int a = 0; a < 3; ++a)
+    if (mu_out) // [pair][m][n] complex, m the left index
+        for (int pi = 0; pi < np; ++pi)
             for (int m = 0; m < M; ++m)
                 for (int n = 0; n < M; ++n) {
-                    const size_t q = (size_t)a * M * M + (size_t)n * M + m;
-                    const size_t o = ((size_t)a * M * M + (size_t)m * M + n) * 2;
-                    mu_out[o] = mu[q * ncomp];
-                    mu_out[o + 1] = ncomp == 2 ? mu[q * ncomp + 1] : 0.0;
+                    const size_t q = (size_t)pi * mm + (size_t)n * M + m;
+                    const size_t o = ((size_t)pi * mm + (size_t)m * M + n) * 2;
+                    mu_out[o] = muz_h[2 * q];
+                    mu_out[o + 1] = muz_h[2 * q + 1];
                 }
-    for (int a = 0; a < 3; ++a)
+    // kernel_native[k][pair] (divided by the sample count, fftcheb_transport.hpp:595-597)
+    std::vector<std::complex<double>> kern((size_t)M * np);
+    for (int pi = 0; pi < np; ++pi) {
+        auto Q = [&](int t, int m, int k) {
+            const size_t o = 2 * (((size_t)pi * 3 + t) * mm + (size_t)k * M + m);
+            return std::complex<double>(Qh[o], Qh[o + 1]);
+        };
         for (int k = 0; k < M; ++k) {
-            const double eps = std::cos(M_PI * (k + 0.5) / M);
-            double acc = 0.0;
-            for (int m = 0; m < M; ++m)
-                acc += W[((size_t)k * M + m) * ncomp] * Qh[((size_t)a * M * M + (size_t)k * M + m) * ncomp];
-            const double native = acc / std::max(1e-14, 1.0 - eps * eps);
-            sigma_out[(size_t)(M - 1 - k) * 3 + a] = prefactor * native / R;
-            if (energy_out && a == 0) energy_out[M - 1 - k] = c + ha * eps;
+            const double th = M_PI * (k + 0.5) / M, eps = std::cos(th);
+            std::complex<double> v(0.0, 0.0);
+            if (formulation == 0) {
+                for (int m = 0; m < M; ++m) v += gw[m] * std::cos(m * th) * Q(0, m, k);
+                v /= std::max(1e-14, 1.0 - eps * eps);
+            } else {
+                // eps (conj(L3) R1 + conj(L1) R3) + i sin (conj(L2) R3 - conj(L3) R2)
+                std::complex<double> l3r1(0.0, 0.0), l1r3(0.0, 0.0), l2r3(0.0, 0.0), l3r2(0.0, 0.0);
+                for (int m = 0; m < M; ++m) {
+                    const double cm = gw[m] * std::cos(m * th);
+                    const std::complex<double> em = gw[m] * std::polar(1.0, -m * th);
+                    l3r1 += cm * Q(0, m, k);
+                    l1r3 += em * Q(1, m, k);
+                    l2r3 += (double)m * em * Q(1, m, k);
+                    l3r2 += cm * Q(2, m, k);
+                }
+                v = eps * (l3r1 + l1r3) + std::complex<double>(0.0, std::sin(th)) * (l2r3 - l3r2);
+            }
+            kern[(size_t)k * np + pi] = v / (double)R;
         }
+    }
+    for (size_t q = 0; q < (size_t)M * 6; ++q) sigma_out[q] = 0.0;
+    std::vector<double> running(6, 0.0), prev(6, 0.0);
+    for (int kk = 0; kk < M; ++kk) { // ascending grid
+        const int src = M - 1 - kk;
+        const double eps_k = std::cos(M_PI * (src + 0.5) / M);
+        if (energy_out) energy_out[kk] = c + ha * eps_k;
+        for (int pi = 0; pi < np; ++pi) {
+            const int comp = pairs[pi];
+            const std::complex<double> v = kern[(size_t)src * np + pi];
+            if (formulation == 0) {
+                sigma_out[(size_t)kk * 6 + comp] = prefactor * v.real();
+            } else {
+                // cumulative trapezoid of Re(i kernel) over ascending eps (:627-652)
+                const double integrand = (std::complex<double>(0.0, 1.0) * v).real();
+                if (kk > 0) {
+                    const double de = eps_k - std::cos(M_PI * (src + 1 + 0.5) / M);
+                    running[comp] += 0.5 * (prev[comp] + integrand) * de;
+                }
+                sigma_out[(size_t)kk * 6 + comp] = prefactor * running[comp];
+                prev[comp] = integrand;
+            }
+        }
+    }
+    return RG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rg_dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor, double *sigma_out,
+                   double *energy_out, double *mu_out) {
+    return dc_spectrum(ctx, M, formulation, kernel, prefactor, sigma_out, energy_out, mu_out);
+}
+
+int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, double *sigma_out,
+                      double *energy_out, double *mu_out) {
+    if (!ctx || !sigma_out || M < 1) return ctx ? fail(ctx, RG_ERR_INVALID, "n_moments >= 1 and sigma_out") : RG_ERR_INVALID;
+    std::vector<double> s6((size_t)M * 6);
+    const int rc = dc_spectrum(ctx, M, 0, kernel, prefactor, s6.data(), energy_out, mu_out);
+    if (rc) return rc;
+    for (int k = 0; k < M; ++k) {
+        sigma_out[(size_t)k * 3 + 0] = s6[(size_t)k * 6 + 0];
+        sigma_out[(size_t)k * 3 + 1] = s6[(size_t)k * 6 + 3];
+        sigma_out[(size_t)k * 3 + 2] = s6[(size_t)k * 6 + 5];
+    }
     return RG_OK;
 }
 

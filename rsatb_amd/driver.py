@@ -311,12 +311,13 @@ class RSATB:
         """compute_dc_spectrum (rsatb/driver.py:366-396, observables.hpp:691-779):
         the zero-temperature Greenwood conductivity on the Chebyshev nodes,
         sigma (n_k, n_moments, 6) in S/m with both spins, components (xx, xy,
-        xz, yy, yz, zz) of which the Greenwood formulation fills the diagonal,
-        and the energies in eV. The left and right sequences are kept whole on
+        xz, yy, yz, zz) (Greenwood fills the diagonal; Bastin all six, as the
+        T = 0 conductivity against the Fermi energy), and the energies in eV. The left and right sequences are kept whole on
         the device (partition_sections and max_workspace_mb are accepted and
         do not apply)."""
-        if formulation != "greenwood":
-            raise RuntimeError("dc formulation must be 'greenwood' on the GPU path (Bastin is not provided)")
+        forms = {"greenwood": 0, "bastin": 1}
+        if formulation not in forms:
+            raise RuntimeError(f"Unknown dc formulation: {formulation}")
         if method not in ("fftcheb", "qe"):
             raise RuntimeError(f"dc method must be 'fftcheb' or 'qe', got '{method}'")
         kernels = {"jackson": _lib.KERNEL_JACKSON, "costaper": _lib.KERNEL_COSTAPER, "none": _lib.KERNEL_NONE}
@@ -330,7 +331,8 @@ class RSATB:
         emin, emax = self.energy_bounds()
         st, e = self.structure, self.edges
         n_rows = st.n_atoms * self.model.norb
-        prefactor = np.pi * 2.0 * 4.0 * n_rows / self.transport_volume(axis)
+        # pi g_s (4 Greenwood, 8 Bastin) N / volume (observables.hpp:722-733)
+        prefactor = np.pi * 2.0 * (4.0 if formulation == "greenwood" else 8.0) * n_rows / self.transport_volume(axis)
         sigma = np.zeros((len(self._kstates), n_moments, 6))
         energy = None
         for k_idx, ks in enumerate(self._kstates):
@@ -339,8 +341,7 @@ class RSATB:
             self._bind_kpoint(k_idx)
             self._ctx.sk_velocity(self.model.arrays("H"), st.atom_type, st.positions, st.cell,
                                   np.stack([e.src, e.dst], axis=1), e.shift)
-            s3, energy = self._ctx.kubo_greenwood(n_moments, kernels[kernel], prefactor)
-            sigma[k_idx][:, [0, 3, 5]] = s3
+            sigma[k_idx], energy = self._ctx.dc_spectrum(n_moments, forms[formulation], kernels[kernel], prefactor)
         # e^2 / (hbar Angstrom) in S/m (rsatb/driver.py:393)
         return sigma * 2.4341348 * 1e6, energy
 

@@ -119,5 +119,33 @@ def test_driver_dc_spectrum():
     ref = kg.greenwood_fftcheb(L, Rs, M, n_samples=3, prefactor=pref)
     assert np.max(np.abs(sigma[0][:, [0, 3, 5]] - ref)) < 1e-10 * np.max(np.abs(ref))
     assert np.allclose(energy, kg.energies(M, c, half), atol=1e-12)
-    with pytest.raises(RuntimeError, match="greenwood"):
-        calc.compute_dc_spectrum(M, formulation="bastin")
+    with pytest.raises(RuntimeError, match="formulation"):
+        calc.compute_dc_spectrum(M, formulation="kubo")
+
+
+@pytest.mark.parametrize("kind", [_lib.SAMPLES_RADEMACHER, _lib.SAMPLES_PHASE])
+def test_bastin_against_oracle(ctx, kind):
+    """rg_dc_spectrum, Bastin: the six cross-axis moment matrices and the
+    integrated spectrum against the oracle's moment form and FFTCheb route."""
+    H, V = setup_si(ctx)
+    e = np.linalg.eigvalsh(H)
+    c, half = 0.5 * (e[0] + e[-1]), 0.55 * (e[-1] - e[0])
+    ctx.set_operator(_lib.FORM_ORTHOGONAL, c, half)
+    R, M = 3, 40
+    ctx.set_samples(H.shape[0], R, kind, 5)
+    print("case: Si spd cell Bastin, samples", kind, flush=True)
+    sigma, energy, mu = ctx.dc_spectrum(M, formulation=1, prefactor=1.7, with_moments=True)
+    psi = ctx.get_samples()
+    L, Rs = kg.sequences(H, V, psi, c, half, M)
+    ref_mu = kg.moment_pairs(L, Rs, M)
+    assert np.max(np.abs(mu - ref_mu)) < 1e-10 * np.max(np.abs(ref_mu))
+    ref = kg.bastin_from_moments(ref_mu, M, n_samples=R, prefactor=1.7)
+    fft = kg.bastin_fftcheb(L, Rs, M, n_samples=R, prefactor=1.7)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(sigma - ref)) < 1e-10 * scale
+    assert np.max(np.abs(sigma - fft)) < 1e-10 * scale
+    # Greenwood through the same entry: the diagonal of the 6-component layout
+    sg, _ = ctx.dc_spectrum(M, formulation=0, prefactor=1.7)
+    g3 = kg.greenwood_from_moments(kg.moment_matrix(L, Rs, M), M, n_samples=R, prefactor=1.7)
+    assert np.max(np.abs(sg[:, [0, 3, 5]] - g3)) < 1e-10 * np.max(np.abs(g3))
+    assert np.all(sg[:, [1, 2, 4]] == 0.0)

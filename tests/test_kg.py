@@ -116,3 +116,26 @@ def test_bond_velocity_hellmann_feynman():
     Vd = kg.bond_velocity(len(pos), 9, ei[0], ei[1], vecs, blocks)
     assert np.allclose(V0, Vd, atol=1e-13)
     assert np.allclose(V0.real, 0.0) and np.allclose(V0, V0.conj().transpose(0, 2, 1), atol=1e-12)
+
+
+def toy_bastin():
+    """make_bastin_problem of test_fftcheb_transport.cpp:115-152."""
+    H = np.array([[0.05, 0.8j], [-0.8j, -0.1]])
+    Vx = np.array([[0.0, 0.9 + 0.2j], [0.9 - 0.2j, 0.0]])
+    Vy = np.array([[0.4, -0.7j], [0.7j, -0.3]])
+    Vz = np.array([[0.2, 0.5], [0.5, -0.25]], dtype=complex)
+    a = np.array([0.6, 0.8j])
+    b = np.array([-0.5 + 0.1j, 0.2 + 0.7j])
+    return H, [Vx, Vy, Vz], a, b
+
+
+def test_toy_bastin_fftcheb_equals_moment_reference():
+    """test_fftcheb_transport.cpp:547-557 (M = 12, window [-1, 1])."""
+    H, V, a, b = toy_bastin()
+    M = 12
+    for r, ns in ((a, 1), (np.stack([a, b], axis=1), 2)):
+        L, R = kg.sequences(H, V, r, 0.0, 1.0, M)
+        mom = kg.bastin_from_moments(kg.moment_pairs(L, R, M), M, n_samples=ns, prefactor=3.0)
+        fft = kg.bastin_fftcheb(L, R, M, n_samples=ns, prefactor=3.0)
+        assert np.max(np.abs(mom - fft)) < 1e-12 * max(1.0, np.max(np.abs(mom)))
+        assert np.all(mom[0] == 0.0)  # the integral starts at the lowest node
