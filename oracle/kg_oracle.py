@@ -242,3 +242,27 @@ def bastin_fftcheb(L, R, M: int, n_samples: int = 1, prefactor: float = 1.0) -> 
         kern[:, q] = eps * (dot(phi["L", a, 3], phi["R", b, 1]) + dot(phi["L", a, 1], phi["R", b, 3])) + \
             1j * np.sin(th) * (dot(phi["L", a, 2], phi["R", b, 3]) - dot(phi["L", a, 3], phi["R", b, 2]))
     return _integrate(kern / n_samples, M, prefactor)
+
+
+def dc_temperature_quadrature(spectrum, energy, mu: float, temp: float, per_segment: int = 4000):
+    """sigma(mu, T) = int -df/dE sigma(E) dE over the grid's span with sigma
+    linear between the grid energies (integrate_dc_spectrum,
+    rsatb/calc/fftcheb_transport.hpp:978-1040), by composite Simpson on a
+    fine subdivision of every segment with the analytic
+    -df/dE = f (1 - f) / kT, kT = 8.617333262e-5 temp eV. An independent route
+    to the closed form the reference uses; temp must be positive."""
+    e = np.asarray(energy, dtype=np.float64)
+    s = np.asarray(spectrum, dtype=np.float64).reshape(len(e), -1)
+    kt = 8.617333262e-5 * temp
+    n = per_segment + (per_segment % 2)
+    out = np.zeros(s.shape[1])
+    for i in range(1, len(e)):
+        x = np.linspace(e[i - 1], e[i], n + 1)
+        z = (x - mu) / kt
+        q = np.exp(-np.abs(z))
+        w = q / (kt * (1.0 + q) ** 2)  # -df/dE = f (1 - f) / kT
+        t = (x - e[i - 1]) / (e[i] - e[i - 1])
+        y = w[:, None] * ((1 - t)[:, None] * s[i - 1] + t[:, None] * s[i])
+        h = (e[i] - e[i - 1]) / n
+        out += h / 3 * (y[0] + y[-1] + 4 * y[1:-1:2].sum(axis=0) + 2 * y[2:-1:2].sum(axis=0))
+    return out

@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import _lib
+from . import _lib, transport
 from .bsr import BSR
 from .lattice import Structure, edges_from_graph, neighbor_edges
 from .models import assemble
@@ -320,6 +320,9 @@ class RSATB:
             raise RuntimeError(f"Unknown dc formulation: {formulation}")
         if method not in ("fftcheb", "qe"):
             raise RuntimeError(f"dc method must be 'fftcheb' or 'qe', got '{method}'")
+        if method == "qe" and formulation != "greenwood":
+            raise RuntimeError("dc method 'qe' supports formulation 'greenwood' only; "
+                               "'bastin' needs the fftcheb row transform")
         kernels = {"jackson": _lib.KERNEL_JACKSON, "costaper": _lib.KERNEL_COSTAPER, "none": _lib.KERNEL_NONE}
         if kernel not in kernels:
             raise RuntimeError(f"unknown kpm kernel '{kernel}'")
@@ -344,6 +347,27 @@ class RSATB:
             sigma[k_idx], energy = self._ctx.dc_spectrum(n_moments, forms[formulation], kernels[kernel], prefactor)
         # e^2 / (hbar Angstrom) in S/m (rsatb/driver.py:393)
         return sigma * 2.4341348 * 1e6, energy
+
+    def compute_dc(self, n_moments: int, mu: float, temp: float, formulation: str = "greenwood",
+                   axis: str = "abc", partition_sections: int = 0, max_workspace_mb: int = 0,
+                   kernel: str = "jackson", method: str = "fftcheb"):
+        """compute_dc (rsatb/driver.py:398-429, observables.hpp:792-854): the
+        conductivity tensor (n_k, 6) in S/m at one chemical potential mu (eV)
+        and temperature temp (K), the spectrum of compute_dc_spectrum
+        integrated against -df/dE per k-point. method "qe" keeps the spectrum
+        only on the nodes within max(20 kT, 3 pi a / n_moments) of mu, zero
+        elsewhere, as the reference's windowed route delivers it (the device
+        evaluates every node either way; the nodes kept are identical)."""
+        sigma, energy = self.compute_dc_spectrum(n_moments, formulation, axis, partition_sections,
+                                                 max_workspace_mb, kernel, method)
+        if method == "qe":
+            emin, emax = self.energy_bounds()
+            nodes = transport.fermi_window_nodes(n_moments, 0.5 * (emax + emin), 0.5 * (emax - emin), mu, temp)
+            keep = np.zeros(n_moments, dtype=bool)
+            keep[nodes] = True
+            sigma = np.where(keep[None, :, None], sigma, 0.0)
+        return np.stack([transport.integrate_dc_spectrum(sigma[k], energy, mu, temp)
+                         for k in range(sigma.shape[0])])
 
     def dump_operator_coo(self, which: str, axis: int = 0, k_idx: int = 0):
         """Global COO triplets of H or S (rsatb/pybind_rsatb.cpp:612-670)."""

@@ -123,6 +123,39 @@ def test_driver_dc_spectrum():
         calc.compute_dc_spectrum(M, formulation="kubo")
 
 
+def test_driver_dc_temperature():
+    """RSATB.compute_dc: the device spectrum integrated against -df/dE at
+    (mu, T) equals the oracle's spectrum integrated by quadrature
+    (observables.hpp:792-854, fftcheb_transport.hpp:978-1040); the windowed
+    "qe" route keeps the nodes near mu and agrees at a temperature whose
+    window holds every node that -df/dE weighs; qe rejects Bastin."""
+    from rsatb_amd.driver import RSATB, SetupOptions
+    from rsatb_amd.lattice import Structure
+    from rsatb_amd.sktb import SktbModel
+    from sk_cases import silicon_params
+
+    basis, h, _ = silicon_params()
+    model = SktbModel(basis, h)
+    pos, cell = silicon_cell()
+    calc = RSATB(Structure(pos, cell), model, SetupOptions(nsamples=2, compute_bounds=True,
+                                                          sample_kind="rademacher"))
+    M = 48
+    sigma_spec, energy = calc.compute_dc_spectrum(M)
+    mu = energy[20] + 0.6 * (energy[21] - energy[20])
+    for temp in (300.0, 3000.0):
+        dc = calc.compute_dc(M, mu, temp)
+        assert dc.shape == (1, 6)
+        ref = kg.dc_temperature_quadrature(sigma_spec[0], energy, mu, temp, per_segment=2000)
+        assert np.max(np.abs(dc[0] - ref)) <= 1e-9 * np.max(np.abs(sigma_spec))
+        qe = calc.compute_dc(M, mu, temp, method="qe")
+        assert np.max(np.abs(qe[0] - dc[0])) <= 1e-6 * np.max(np.abs(sigma_spec))
+    dc0 = calc.compute_dc(M, mu, 0.0)
+    t = (mu - energy[20]) / (energy[21] - energy[20])
+    np.testing.assert_allclose(dc0[0], (1 - t) * sigma_spec[0, 20] + t * sigma_spec[0, 21], rtol=1e-12, atol=0)
+    with pytest.raises(RuntimeError, match="greenwood"):
+        calc.compute_dc(M, mu, 300.0, formulation="bastin", method="qe")
+
+
 @pytest.mark.parametrize("kind", [_lib.SAMPLES_RADEMACHER, _lib.SAMPLES_PHASE])
 def test_bastin_against_oracle(ctx, kind):
     """rg_dc_spectrum, Bastin: the six cross-axis moment matrices and the
