@@ -67,7 +67,13 @@ typedef struct rg_context rg_context;
 #define RG_SLOT_VX 4
 #define RG_SLOT_VY 5
 #define RG_SLOT_VZ 6
-#define RG_SLOT_COUNT 7
+/* tangent direction of rg_dc_spectrum_tangent: dH and dB_a (real, H's
+ * block rows and block size; any pattern) */
+#define RG_SLOT_DH 7
+#define RG_SLOT_DVX 8
+#define RG_SLOT_DVY 9
+#define RG_SLOT_DVZ 10
+#define RG_SLOT_COUNT 11
 
 /* value types of an uploaded matrix */
 #define RG_F64 0
@@ -275,6 +281,20 @@ RG_API int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefacto
  * Greenwood, 6 in component order for Bastin. */
 RG_API int rg_dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor,
                           double *sigma_out, double *energy_out, double *mu_out);
+
+/* rg_dc_spectrum and its directional derivative: dsigma_out[k * 6 + comp] =
+ * d sigma / d e along H -> H + e dH, B_a -> B_a + e dB_a (RG_SLOT_DH,
+ * RG_SLOT_DVX..), with the window of rg_set_operator held fixed. The
+ * tangent recurrence yd_n = 2 (dH / a) y_{n-1} + 2 A yd_{n-1} - yd_{n-2} runs
+ * beside each stored sequence, and dmu = s^2 (dL^H R + L^H dR). For a
+ * parameterization H(theta) = sum_k theta_k H_k (and B_a likewise) one call
+ * per parameter gives the Jacobian d sigma / d theta, the derivative behind
+ * gradients of conductivity losses. No counterpart in the reference (its
+ * transport path has no derivatives); the spectrum it differentiates is
+ * dc_spectrum_fftcheb (fftcheb_transport.hpp:543-653). Memory: 12 M x rows x R
+ * vector elements. */
+RG_API int rg_dc_spectrum_tangent(rg_context *ctx, int M, int formulation, int kernel, double prefactor,
+                                  double *sigma_out, double *dsigma_out, double *energy_out);
 
 /* ---- graph-local S^{-1/2} (graph_inverse_sqrt seed, rsatb/calc/isqrtm.hpp:137-232;
  *      ISQRTMPreconditioner ctor, rsatb/calc/preconditioner.hpp:184-192) ---- */

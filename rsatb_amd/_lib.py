@@ -23,6 +23,7 @@ RG_ERR_UNSUPPORTED = 6
 
 SLOT_H, SLOT_Z, SLOT_ZH, SLOT_S = 0, 1, 2, 3
 SLOT_VX, SLOT_VY, SLOT_VZ = 4, 5, 6
+SLOT_DH, SLOT_DVX, SLOT_DVY, SLOT_DVZ = 7, 8, 9, 10
 F64, C128 = 0, 2
 FORM_ORTHOGONAL, FORM_SANDWICH = 0, 1
 SAMPLES_RADEMACHER, SAMPLES_PHASE, SAMPLES_UPLOAD_REAL, SAMPLES_UPLOAD_COMPLEX = 0, 1, 2, 3
@@ -76,6 +77,7 @@ SIGNATURES = {
     "rg_sk_velocity": (_i, [_vp, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32]),
     "rg_kubo_greenwood": (_i, [_vp, _i, _i, _d, _pd, _pd, _pd]),
     "rg_dc_spectrum": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
+    "rg_dc_spectrum_tangent": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
     "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_set_checkpointing": (_i, [_vp, _i]),
@@ -253,6 +255,19 @@ class Context:
                                       _dptr(energy), _dptr(mu.view(np.float64)) if with_moments else None)
         _check(self._h, rc, "rg_dc_spectrum")
         return (sigma, energy, mu) if with_moments else (sigma, energy)
+
+    def dc_spectrum_tangent(self, n_moments: int, formulation: int = 0, kernel: int = KERNEL_JACKSON,
+                            prefactor: float = 1.0):
+        """(sigma [M, 6], dsigma [M, 6], energies [M]): the spectrum and its
+        derivative along the tangent matrices in SLOT_DH, SLOT_DVX..DVZ."""
+        M = int(n_moments)
+        sigma = np.zeros((M, 6))
+        dsigma = np.zeros((M, 6))
+        energy = np.zeros(M)
+        rc = self._lib.rg_dc_spectrum_tangent(self._h, M, int(formulation), int(kernel), float(prefactor),
+                                              _dptr(sigma), _dptr(dsigma), _dptr(energy))
+        _check(self._h, rc, "rg_dc_spectrum_tangent")
+        return sigma, dsigma, energy
 
     def _sk_call(self, velocity: bool, slot: int, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift,
                  kpoint):

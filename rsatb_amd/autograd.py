@@ -1,5 +1,6 @@
 """Differentiable KPM moments: a torch.autograd.Function over the library's
-adjoint Chebyshev recurrence (rg_kpm_moments_backward).
+adjoint Chebyshev recurrence (rg_kpm_moments_backward); and a differentiable
+DC conductivity spectrum over its tangent recurrence (rg_dc_spectrum_tangent).
 
 PyTorch is used only here, as the autograd wrapper: the forward and the
 backward both run in the gfx950 library; torch sees the moments (a real
@@ -73,6 +74,116 @@ def sk_pz_parameterization(st: Structure, edges: Edges, shell_radii) -> LinearOp
     return LinearOperatorModel(pattern, np.stack(cols_basis, axis=1), names)
 
 
+@dataclass
+class LinearTransportModel:
+    """H(theta) = sum_k theta_k H_k with the bond-moment velocity factors
+    B_a(theta) = sum_k theta_k B_{a,k} (V_a = i B_a; B_{a,k} holds the bond
+    vector's a component times H_k's edge entries, onsite entries 0), on one
+    pattern. Built by linear_transport_model."""
+
+    hamiltonian: LinearOperatorModel
+    velocity: tuple  # three LinearOperatorModel, one per axis
+
+
+def linear_transport_model(n_atoms: int, edges: Edges, edge_coef, onsite_coef, names) -> LinearTransportModel:
+    """One parameter k per column: edge_coef [E, P] is theta_k's hopping
+    coefficient on each directed edge (src -> dst, the p_z entry H_ij), and
+    onsite_coef [n_atoms, P] its onsite coefficient. Entries of periodic
+    images that land on one (i, j) are summed, for H and for every B_a with
+    its own bond vector, as the velocity of an orthogonal basis at Gamma
+    requires (rsatb/operator/commutator_velocity.hpp:7-43)."""
+    ec = np.asarray(edge_coef, dtype=np.float64).reshape(edges.src.size, -1)
+    oc = np.asarray(onsite_coef, dtype=np.float64).reshape(n_atoms, -1)
+    if ec.shape[1] != oc.shape[1] or ec.shape[1] != len(names):
+        raise RuntimeError("edge and onsite coefficients need one column per parameter name")
+    rows = np.concatenate([np.arange(n_atoms), edges.src])
+    cols = np.concatenate([np.arange(n_atoms), edges.dst])
+    weights = [None, edges.vec[:, 0], edges.vec[:, 1], edges.vec[:, 2]]
+    models = []
+    pattern = None
+    for w in weights:
+        basis = []
+        for k in range(ec.shape[1]):
+            if w is None:
+                coef = np.concatenate([oc[:, k], ec[:, k]])
+            else:
+                coef = np.concatenate([np.zeros(n_atoms), ec[:, k] * w])
+            Hk = BSR.from_blocks(n_atoms, 1, rows, cols, coef.reshape(-1, 1, 1))
+            pattern = Hk if pattern is None else pattern
+            if not (np.array_equal(Hk.row_ptr, pattern.row_ptr) and np.array_equal(Hk.col_ind, pattern.col_ind)):
+                raise RuntimeError("parameter patterns differ")
+            basis.append(Hk.values.reshape(-1))
+        models.append(LinearOperatorModel(pattern, np.stack(basis, axis=1), tuple(names)))
+    return LinearTransportModel(models[0], tuple(models[1:]))
+
+
+class DifferentiableConductivity:
+    """The DC spectrum sigma(theta) (dc_spectrum_fftcheb,
+    rsatb/calc/fftcheb_transport.hpp:543-653, orthogonal form at Gamma) and
+    its Jacobian d sigma / d theta, one tangent pass per parameter
+    (rg_dc_spectrum_tangent): forward-mode, which for the few parameters of a
+    hopping parameterization costs (1 + P) spectra instead of an adjoint's
+    stored cotangent sequences. The window is held fixed."""
+
+    def __init__(self, model: LinearTransportModel, nsamples: int, window, n_moments: int,
+                 formulation: str = "greenwood", kernel: str = "jackson", prefactor: float = 1.0,
+                 sample_kind: str = "rademacher", seed: int = 20240917, device: int = 0):
+        forms = {"greenwood": 0, "bastin": 1}
+        kernels = {"jackson": _lib.KERNEL_JACKSON, "costaper": _lib.KERNEL_COSTAPER, "none": _lib.KERNEL_NONE}
+        if formulation not in forms:
+            raise RuntimeError(f"Unknown dc formulation: {formulation}")
+        if kernel not in kernels:
+            raise RuntimeError(f"unknown kpm kernel '{kernel}'")
+        emin, emax = window
+        if not (math.isfinite(emin) and math.isfinite(emax) and emin < emax):
+            raise RuntimeError("window must be finite with emin < emax")
+        self.model = model
+        self.M = int(n_moments)
+        self.formulation = forms[formulation]
+        self.kernel = kernels[kernel]
+        self.prefactor = float(prefactor)
+        self.ctx = _lib.Context(device)
+        kind = _lib.SAMPLES_PHASE if sample_kind == "phase" else _lib.SAMPLES_RADEMACHER
+        self.ctx.set_samples(model.hamiltonian.pattern.n_rows, nsamples, kind, seed)
+        self.ctx.set_operator(_lib.FORM_ORTHOGONAL, (emax + emin) / 2.0, (emax - emin) / 2.0)
+        self.energy = None
+
+    def _bind(self, theta):
+        self.ctx.upload_bsr(_lib.SLOT_H, self.model.hamiltonian.build(theta))
+        for a in range(3):
+            self.ctx.upload_bsr(_lib.SLOT_VX + a, self.model.velocity[a].build(theta))
+
+    def spectrum(self, theta) -> np.ndarray:
+        """sigma [M, 6] on the ascending nodes (energies in self.energy)."""
+        theta = np.asarray(theta, dtype=np.float64)
+        self._bind(theta)
+        sigma, self.energy = self.ctx.dc_spectrum(self.M, self.formulation, self.kernel, self.prefactor)
+        return sigma
+
+    def jacobian(self, theta):
+        """(sigma [M, 6], d sigma / d theta [P, M, 6])."""
+        theta = np.asarray(theta, dtype=np.float64)
+        self._bind(theta)
+        P = theta.size
+        jac = np.zeros((P, self.M, 6))
+        sigma = None
+        for k in range(P):
+            e_k = np.zeros(P)
+            e_k[k] = 1.0
+            self.ctx.upload_bsr(_lib.SLOT_DH, self.model.hamiltonian.build(e_k))
+            for a in range(3):
+                self.ctx.upload_bsr(_lib.SLOT_DVX + a, self.model.velocity[a].build(e_k))
+            sigma, jac[k], self.energy = self.ctx.dc_spectrum_tangent(self.M, self.formulation, self.kernel,
+                                                                      self.prefactor)
+        if sigma is None:
+            sigma = self.spectrum(theta)
+        return sigma, jac
+
+    def conductivity(self, theta):
+        """torch entry point: sigma(theta) [M, 6] with autograd."""
+        return _cond_fn().apply(theta, self)
+
+
 class DifferentiableKPM:
     """Moments mu(theta) on one device context with a backward pass."""
 
@@ -142,6 +253,31 @@ def _kpm_fn():
 
     _FN = KPMMoments
     return _FN
+
+
+_COND_FN = None
+
+
+def _cond_fn():
+    global _COND_FN
+    if _COND_FN is not None:
+        return _COND_FN
+    import torch
+
+    class DcSpectrum(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, engine):
+            sigma, jac = engine.jacobian(theta.detach().cpu().double().numpy())
+            ctx.jac = jac
+            return torch.from_numpy(sigma)
+
+        @staticmethod
+        def backward(ctx, grad_sigma):
+            g = np.tensordot(ctx.jac, grad_sigma.detach().cpu().double().numpy(), axes=([1, 2], [0, 1]))
+            return torch.from_numpy(g), None
+
+    _COND_FN = DcSpectrum
+    return _COND_FN
 
 
 def kpm_dos_torch(mu, x, edge_inset: float):

@@ -52,6 +52,13 @@ __global__ __launch_bounds__(256) void k_zero(double *__restrict__ y, int64_t n)
         y[t] = 0.0;
 }
 
+__global__ __launch_bounds__(256) void k_axpby(double *__restrict__ y, const double *__restrict__ x,
+                                               double alpha, double beta, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        y[t] = alpha * x[t] + beta * y[t];
+}
+
 int grid_for(int64_t n) {
     int64_t g = (n + 255) / 256;
     if (g > 16384) g = 16384;
@@ -71,6 +78,11 @@ hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *ro
 
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(256), 0, st, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(double *y, const double *x, double alpha, double beta, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(256), 0, st, y, x, alpha, beta, n);
     return hipGetLastError();
 }
 

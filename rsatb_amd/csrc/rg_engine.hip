@@ -102,7 +102,7 @@ struct rg_context {
     // edges, edge blocks, flags, gather plan
     DevBuf sk_ints, sk_dbls, sk_type, sk_pos, sk_ei, sk_sh, sk_eblk, sk_flags, sk_sp, sk_src, sk_ph, sk_row;
     // Kubo-Greenwood: stored left/right sequences and the moment matrices
-    DevBuf kg_store, kg_mu;
+    DevBuf kg_store, kg_mu, kg_work;
     rocblas_handle blas = nullptr;
 
     std::atomic<int> cancel{0};
@@ -1254,8 +1254,9 @@ namespace {
 // diagonal pairs; Bastin (1) all six from mu^{ab}, a <= b. mu_out: [pairs][M][M]
 // complex, pairs 3 (Greenwood: xx, yy, zz) or 6 (Bastin, component order).
 int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor, double *sigma_out,
-                double *energy_out, double *mu_out) {
+                double *energy_out, double *mu_out, double *dsigma_out = nullptr) {
     if (!ctx || !sigma_out) return RG_ERR_INVALID;
+    const bool tangent = dsigma_out != nullptr;
     if (formulation != 0 && formulation != 1) return fail(ctx, RG_ERR_INVALID, "formulation: 0 Greenwood, 1 Bastin");
     int rc = check_operator(ctx);
     if (rc) return rc;
@@ -1273,6 +1274,13 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
         if (V.nbr != H.nbr || V.b != H.b) return fail(ctx, RG_ERR_INVALID, "velocity and H differ in shape");
         if (V.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "velocity factors must be real (Gamma)");
     }
+    if (tangent)
+        for (int q = RG_SLOT_DH; q <= RG_SLOT_DVZ; ++q) {
+            const Slot &D = ctx->slots[q];
+            if (!D.present) return fail(ctx, RG_ERR_STATE, "tangent slots RG_SLOT_DH .. RG_SLOT_DVZ are empty");
+            if (D.nbr != H.nbr || D.b != H.b) return fail(ctx, RG_ERR_INVALID, "tangent matrices and H differ in shape");
+            if (D.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "tangent matrices must be real (Gamma)");
+        }
     const int64_t rows = op_rows(ctx);
     if (ctx->n_rows != rows) return fail(ctx, RG_ERR_INVALID, "sample rows differ from operator rows");
     RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
@@ -1283,11 +1291,15 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     const size_t eb = vec_elem_bytes(vm), K = (size_t)rows * R, nb = K * eb;
     for (int i = 0; i < 3; ++i) RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
     RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
-    // [left | right][axis][moment] vectors of K elements
-    RG_HIP(ctx, ctx->kg_store.ensure(6 * (size_t)M * nb), "hipMalloc sequence store");
+    // [left | right | left tangent | right tangent][axis][moment] vectors of K elements
+    RG_HIP(ctx, ctx->kg_store.ensure((tangent ? 12 : 6) * (size_t)M * nb), "hipMalloc sequence store");
+    if (tangent) RG_HIP(ctx, ctx->kg_work.ensure(4 * nb), "hipMalloc tangent work vectors");
     char *store = static_cast<char *>(ctx->kg_store.p);
     auto Lst = [&](int a, int m) { return (void *)(store + ((size_t)a * M + m) * nb); };
     auto Rst = [&](int a, int m) { return (void *)(store + ((size_t)(3 + a) * M + m) * nb); };
+    auto Ldt = [&](int a, int m) { return (void *)(store + ((size_t)(6 + a) * M + m) * nb); };
+    auto Rdt = [&](int a, int m) { return (void *)(store + ((size_t)(9 + a) * M + m) * nb); };
+    const int64_t nvec = (int64_t)K * (vm.complex_vec ? 2 : 1); // doubles per vector
 
     const double ha = ctx->half_width, c = ctx->center;
     EpiArgs plain, first, step;
@@ -1341,6 +1353,86 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     }
     if (rc) return rc;
 
+    if (tangent) {
+        // Directional derivative along H -> H + e dH, B_a -> B_a + e dB_a
+        // (slots RG_SLOT_DH, RG_SLOT_DVX..), the window held fixed, so that
+        // dA = dH / a. For y_n = T_n(A) y_0 the tangent obeys
+        //   yd_1 = dA y_0 + A yd_0,   yd_n = 2 dA y_{n-1} + 2 A yd_{n-1} - yd_{n-2}
+        // (the recurrence differentiated term by term); then
+        //   dL_m^a = dB_a T_m r + B_a (T_m r)',   dR_0^a = dB_a r,
+        //   dmu^{ab} = s^2 (dL^a^H R^b + L^a^H dR^b).
+        char *wk = static_cast<char *>(ctx->kg_work.p);
+        void *tmp = wk + 3 * nb;
+        auto vaxpby = [&](void *y, const void *x, double alpha, double beta) {
+            return launch_axpby(static_cast<double *>(y), static_cast<const double *>(x), alpha, beta, nvec,
+                                ctx->stream) == hipSuccess
+                       ? RG_OK
+                       : fail(ctx, RG_ERR_HIP, "axpby launch");
+        };
+        // yd_n from y_{n-1} (primal), yd_{n-1}, yd_{n-2}; n >= 1
+        auto tstep = [&](int n, const void *y_prev, const void *yd_prev, const void *yd_prev2, void *yd_out) {
+            int r = spmm(ctx, RG_SLOT_DH, vm, y_prev, tmp, R, plain, nullptr); // tmp = dH y_{n-1}
+            if (r) return r;
+            if (n == 1) {
+                r = apply_chain(ctx, vm, yd_prev, yd_out, R, first, &g);
+                return r ? r : vaxpby(yd_out, tmp, 1.0 / ha, 1.0);
+            }
+            r = vaxpby(tmp, yd_prev2, 1.0, -2.0 / ha); // tmp = yd_{n-2} - 2 dA y_{n-1}
+            if (r) return r;
+            EpiArgs st2 = step;
+            st2.xp = tmp;
+            return apply_chain(ctx, vm, yd_prev, yd_out, R, st2, &g);
+        };
+        auto cancelled = [&]() {
+            if (!ctx->cancel.load()) return RG_OK;
+            (void)hipStreamSynchronize(ctx->stream);
+            drain_events(ctx);
+            return fail(ctx, RG_ERR_CANCELLED, "interrupted");
+        };
+        // right tangents: Rd_0 = dB_a r, Rd_n along the stored R_n
+        void *r0 = t2; // t2 is free after the left pass; restage the samples
+        rc = stage_samples(ctx, vm, r0);
+        for (int a = 0; a < 3 && !rc; ++a) {
+            rc = spmm(ctx, RG_SLOT_DVX + a, vm, r0, Rdt(a, 0), R, plain, nullptr);
+            for (int n = 1; n < M && !rc; ++n) {
+                rc = tstep(n, Rst(a, n - 1), Rdt(a, n - 1), n >= 2 ? Rdt(a, n - 2) : nullptr, Rdt(a, n));
+                if (!rc) rc = cancelled();
+            }
+        }
+        // left tangents: the base chain x_m = T_m r again with its tangent
+        // xd_m (xd_0 = 0), rolling in t0..t2 and three work vectors
+        void *x0 = t0, *x1 = t1, *x2 = t2;
+        void *d0 = wk, *d1 = wk + nb, *d2 = wk + 2 * nb;
+        if (!rc) rc = stage_samples(ctx, vm, x0);
+        if (!rc) RG_HIP(ctx, launch_zero(static_cast<double *>(d0), nvec, ctx->stream), "zero launch");
+        for (int m = 0; m < M && !rc; ++m) {
+            void *xm = x0, *dm = d0;
+            if (m == 1) {
+                rc = apply_chain(ctx, vm, x0, x1, R, first, &g);
+                if (!rc) rc = tstep(1, x0, d0, nullptr, d1);
+                xm = x1;
+                dm = d1;
+            } else if (m >= 2) {
+                step.xp = x0;
+                rc = apply_chain(ctx, vm, x1, x2, R, step, &g);
+                if (!rc) rc = tstep(m, x1, d1, d0, d2);
+                std::swap(x0, x1);
+                std::swap(x1, x2);
+                std::swap(d0, d1);
+                std::swap(d1, d2);
+                xm = x1;
+                dm = d1;
+            }
+            for (int a = 0; a < 3 && !rc; ++a) {
+                rc = spmm(ctx, RG_SLOT_DVX + a, vm, xm, Ldt(a, m), R, plain, nullptr);
+                if (!rc) rc = spmm(ctx, RG_SLOT_VX + a, vm, dm, tmp, R, plain, nullptr);
+                if (!rc) rc = vaxpby(Ldt(a, m), tmp, 1.0, 1.0);
+            }
+            if (!rc) rc = cancelled();
+        }
+        if (rc) return rc;
+    }
+
     // mu^{ab} = s^2 L~^a^H R~^b, s = 1 / (2 a): one GEMM per axis pair, K = rows R
     if (!ctx->blas) {
         if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
@@ -1358,13 +1450,17 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     // device layout: mu as computed, then complex mu (Bastin or complex
     // samples), three complex weight matrices, three complex Q per pair
     const size_t zmat = mm * 2;
-    RG_HIP(ctx, ctx->kg_mu.ensure((mu_real + (size_t)np * zmat + 3 * zmat + 3 * (size_t)np * zmat) * 8),
+    RG_HIP(ctx,
+           ctx->kg_mu.ensure(((tangent ? 2 : 1) * mu_real + (size_t)np * zmat + 3 * zmat + 3 * (size_t)np * zmat) * 8),
            "hipMalloc moment matrices");
     double *mud = static_cast<double *>(ctx->kg_mu.p);
     double *muz = mud + mu_real;
     double *Wz = muz + (size_t)np * zmat;
     double *Qz = Wz + 3 * zmat;
+    double *dmud = Qz + 3 * (size_t)np * zmat; // tangent moments (mu_real doubles)
     const double s2 = 1.0 / (4.0 * ha * ha);
+    // out[pi] (+)= s^2 Lseq(a)^H Rseq(b) for every pair
+    auto moment_gemms = [&](auto Lseq, auto Rseq, double *out, double beta_in) -> int {
     for (int pi = 0; pi < np; ++pi) {
         const int a = comp_a[pairs[pi]], b = comp_b[pairs[pi]];
         rocblas_status st;
@@ -1376,17 +1472,17 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
             RG_HIP(ctx, hipEventRecord(ev0, ctx->stream), "hipEventRecord");
         }
         if (vm.complex_vec) {
-            const rocblas_double_complex alpha{s2, 0.0}, beta{0.0, 0.0};
+            const rocblas_double_complex alpha{s2, 0.0}, beta{beta_in, 0.0};
             st = rocblas_zgemm(ctx->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, M, M,
-                               (rocblas_int)K, &alpha, static_cast<const rocblas_double_complex *>(Lst(a, 0)),
-                               (rocblas_int)K, static_cast<const rocblas_double_complex *>(Rst(b, 0)),
-                               (rocblas_int)K, &beta, reinterpret_cast<rocblas_double_complex *>(mud + pi * zmat),
+                               (rocblas_int)K, &alpha, static_cast<const rocblas_double_complex *>(Lseq(a, 0)),
+                               (rocblas_int)K, static_cast<const rocblas_double_complex *>(Rseq(b, 0)),
+                               (rocblas_int)K, &beta, reinterpret_cast<rocblas_double_complex *>(out + pi * zmat),
                                M);
         } else {
-            const double alpha = s2, beta = 0.0;
+            const double alpha = s2, beta = beta_in;
             st = rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, M, M, (rocblas_int)K,
-                               &alpha, static_cast<const double *>(Lst(a, 0)), (rocblas_int)K,
-                               static_cast<const double *>(Rst(b, 0)), (rocblas_int)K, &beta, mud + pi * mm, M);
+                               &alpha, static_cast<const double *>(Lseq(a, 0)), (rocblas_int)K,
+                               static_cast<const double *>(Rseq(b, 0)), (rocblas_int)K, &beta, out + pi * mm, M);
         }
         if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "moment GEMM failed");
         if (ctx->prof) {
@@ -1398,6 +1494,12 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
             ctx->stats[RG_KCLASS_KG_GEMM].flops = 2.0 * (double)mm * (double)K * (vm.complex_vec ? 4.0 : 1.0);
         }
     }
+    return RG_OK;
+    };
+    rc = moment_gemms(Lst, Rst, mud, 0.0);
+    if (!rc && tangent) rc = moment_gemms(Ldt, Rst, dmud, 0.0);
+    if (!rc && tangent) rc = moment_gemms(Lst, Rdt, dmud, 1.0);
+    if (rc) return rc;
     // node weights (complex, column-major [n + k M]), theta_k = pi (k + 1/2) / M:
     //   GC[n,k] = g_n cos(n theta_k), GE[n,k] = g_n e^{i n theta_k}, GNE = n GE,
     //   g_n = 2 j_n / (pi (1 + delta_n0))    (fftcheb_transport.hpp:259-273)
@@ -1416,9 +1518,12 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     }
     RG_HIP(ctx, hipMemcpyAsync(Wz, Wh.data(), Wh.size() * 8, hipMemcpyHostToDevice, ctx->stream),
            "upload node weights");
+    // sigma from one set of moments (linear in them: the tangent moments give
+    // the tangent spectrum)
+    auto finish = [&](const double *mu_dev, double *sig_out, double *muh_out) -> int {
     // complex copies of mu (a real mu widened on the host)
     std::vector<double> mu_h(mu_real);
-    RG_HIP(ctx, hipMemcpyAsync(mu_h.data(), mud, mu_real * 8, hipMemcpyDeviceToHost, ctx->stream),
+    RG_HIP(ctx, hipMemcpyAsync(mu_h.data(), mu_dev, mu_real * 8, hipMemcpyDeviceToHost, ctx->stream),
            "download moments");
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     std::vector<double> muz_h((size_t)np * zmat);
@@ -1445,14 +1550,14 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     RG_HIP(ctx, hipMemcpyAsync(Qh.data(), Qz, Qh.size() * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (ctx->prof) drain_events(ctx);
-    if (mu_out) // [pair][m][n] complex, m the left index
+    if (muh_out) // [pair][m][n] complex, m the left index
         for (int pi = 0; pi < np; ++pi)
             for (int m = 0; m < M; ++m)
                 for (int n = 0; n < M; ++n) {
                     const size_t q = (size_t)pi * mm + (size_t)n * M + m;
                     const size_t o = ((size_t)pi * mm + (size_t)m * M + n) * 2;
-                    mu_out[o] = muz_h[2 * q];
-                    mu_out[o + 1] = muz_h[2 * q + 1];
+                    muh_out[o] = muz_h[2 * q];
+                    muh_out[o + 1] = muz_h[2 * q + 1];
                 }
     // kernel_native[k][pair] (divided by the sample count, fftcheb_transport.hpp:595-597)
     std::vector<std::complex<double>> kern((size_t)M * np);
@@ -1483,7 +1588,7 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
             kern[(size_t)k * np + pi] = v / (double)R;
         }
     }
-    for (size_t q = 0; q < (size_t)M * 6; ++q) sigma_out[q] = 0.0;
+    for (size_t q = 0; q < (size_t)M * 6; ++q) sig_out[q] = 0.0;
     std::vector<double> running(6, 0.0), prev(6, 0.0);
     for (int kk = 0; kk < M; ++kk) { // ascending grid
         const int src = M - 1 - kk;
@@ -1493,7 +1598,7 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
             const int comp = pairs[pi];
             const std::complex<double> v = kern[(size_t)src * np + pi];
             if (formulation == 0) {
-                sigma_out[(size_t)kk * 6 + comp] = prefactor * v.real();
+                sig_out[(size_t)kk * 6 + comp] = prefactor * v.real();
             } else {
                 // cumulative trapezoid of Re(i kernel) over ascending eps (:627-652)
                 const double integrand = (std::complex<double>(0.0, 1.0) * v).real();
@@ -1501,12 +1606,16 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
                     const double de = eps_k - std::cos(M_PI * (src + 1 + 0.5) / M);
                     running[comp] += 0.5 * (prev[comp] + integrand) * de;
                 }
-                sigma_out[(size_t)kk * 6 + comp] = prefactor * running[comp];
+                sig_out[(size_t)kk * 6 + comp] = prefactor * running[comp];
                 prev[comp] = integrand;
             }
         }
     }
     return RG_OK;
+    };
+    rc = finish(mud, sigma_out, mu_out);
+    if (!rc && tangent) rc = finish(dmud, dsigma_out, nullptr);
+    return rc;
 }
 
 } // namespace
@@ -1516,6 +1625,13 @@ extern "C" {
 int rg_dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor, double *sigma_out,
                    double *energy_out, double *mu_out) {
     return dc_spectrum(ctx, M, formulation, kernel, prefactor, sigma_out, energy_out, mu_out);
+}
+
+int rg_dc_spectrum_tangent(rg_context *ctx, int M, int formulation, int kernel, double prefactor,
+                           double *sigma_out, double *dsigma_out, double *energy_out) {
+    if (!ctx) return RG_ERR_INVALID;
+    if (!sigma_out || !dsigma_out) return fail(ctx, RG_ERR_INVALID, "sigma_out and dsigma_out are required");
+    return dc_spectrum(ctx, M, formulation, kernel, prefactor, sigma_out, energy_out, nullptr, dsigma_out);
 }
 
 int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefactor, double *sigma_out,

@@ -134,6 +134,8 @@ hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *ro
                              bool cplx, double coef, double *G, hipStream_t st);
 // y = 0 over n doubles
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st);
+// y = alpha x + beta y over n doubles
+hipError_t launch_axpby(double *y, const double *x, double alpha, double beta, int64_t n, hipStream_t st);
 
 // ---- Slater-Koster assembly (rg_sk.hip) ----
 // Device copy of one matrix's SK table; channels grouped by (row type, col

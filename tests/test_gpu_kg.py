@@ -182,3 +182,93 @@ def test_bastin_against_oracle(ctx, kind):
     g3 = kg.greenwood_from_moments(kg.moment_matrix(L, Rs, M), M, n_samples=R, prefactor=1.7)
     assert np.max(np.abs(sg[:, [0, 3, 5]] - g3)) < 1e-10 * np.max(np.abs(g3))
     assert np.all(sg[:, [1, 2, 4]] == 0.0)
+
+
+def _oracle_spectrum(model, theta, psi, c, half, M, formulation, n_samples):
+    H = model.hamiltonian.build(theta).to_dense()
+    V = np.stack([1j * model.velocity[a].build(theta).to_dense() for a in range(3)])
+    L, Rs = kg.sequences(H, V, psi, c, half, M)
+    if formulation == "greenwood":
+        return kg.greenwood_fftcheb(L, Rs, M, n_samples=n_samples)
+    return kg.bastin_fftcheb(L, Rs, M, n_samples=n_samples)
+
+
+@pytest.mark.parametrize("formulation", ["greenwood", "bastin"])
+def test_dc_tangent_against_oracle_finite_differences(formulation):
+    """rg_dc_spectrum_tangent: the Jacobian d sigma / d theta of a graphene
+    p_z spectrum (onsite and three hopping shells, velocity from the same
+    parameters) equals central differences of the oracle's spectrum
+    (h = 1e-5, fixed window) to 1e-6 of its largest entry, and the spectrum
+    equals the oracle's to 1e-10."""
+    from rsatb_amd.autograd import DifferentiableConductivity
+    from transport_cases import THETA, graphene_transport
+
+    st, e, model, ec, oc = graphene_transport(4, 4)
+    H = model.hamiltonian.build(THETA).to_dense()
+    a = 1.3 * np.max(np.abs(np.linalg.eigvalsh(H)))
+    M, S = 40, 3
+    eng = DifferentiableConductivity(model, S, (-a, a), M, formulation=formulation)
+    sigma, jac = eng.jacobian(THETA)
+    psi = eng.ctx.get_samples()
+    comps = [0, 3, 5] if formulation == "greenwood" else list(range(6))
+    ref = _oracle_spectrum(model, THETA, psi, 0.0, a, M, formulation, S)
+    assert np.max(np.abs(sigma[:, comps] - ref)) <= 1e-10 * np.max(np.abs(ref))
+    h = 1e-5
+    for k in range(THETA.size):
+        tp, tm = THETA.copy(), THETA.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (_oracle_spectrum(model, tp, psi, 0.0, a, M, formulation, S)
+              - _oracle_spectrum(model, tm, psi, 0.0, a, M, formulation, S)) / (2 * h)
+        scale = max(np.max(np.abs(fd)), 1e-30)
+        assert np.max(np.abs(jac[k][:, comps] - fd)) <= 1e-6 * scale, (k, np.max(np.abs(jac[k][:, comps] - fd)), scale)
+    # every call recomputes the same primal spectrum
+    np.testing.assert_allclose(eng.spectrum(THETA), sigma, rtol=0, atol=1e-12 * np.max(np.abs(sigma)))
+
+
+def test_dc_conductivity_torch_gradient():
+    """DifferentiableConductivity.conductivity under torch autograd: the
+    gradient of L = sum_k w_k sigma_xx(E_k) + sigma_yy(E_mid)^2 equals
+    central differences of L through the device spectrum."""
+    import torch
+
+    from rsatb_amd.autograd import DifferentiableConductivity
+    from transport_cases import THETA, graphene_transport
+
+    st, e, model, ec, oc = graphene_transport(4, 4)
+    H = model.hamiltonian.build(THETA).to_dense()
+    a = 1.3 * np.max(np.abs(np.linalg.eigvalsh(H)))
+    M = 32
+    eng = DifferentiableConductivity(model, 2, (-a, a), M, sample_kind="phase")
+    w = torch.from_numpy(np.random.default_rng(3).standard_normal(M))
+
+    def loss_of(sig):
+        return (w * sig[:, 0]).sum() + sig[M // 2, 3] ** 2
+
+    theta = torch.tensor(THETA, requires_grad=True)
+    loss = loss_of(eng.conductivity(theta))
+    loss.backward()
+    g = theta.grad.numpy()
+    h = 1e-5
+    fd = np.zeros_like(THETA)
+    for k in range(THETA.size):
+        tp, tm = THETA.copy(), THETA.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (loss_of(torch.from_numpy(eng.spectrum(tp))) - loss_of(torch.from_numpy(eng.spectrum(tm)))).item() / (2 * h)
+    assert np.max(np.abs(g - fd)) <= 1e-6 * np.max(np.abs(fd))
+
+
+def test_dc_tangent_needs_tangent_slots():
+    """The tangent entry point rejects missing tangent matrices."""
+    from transport_cases import THETA, graphene_transport
+
+    st, e, model, ec, oc = graphene_transport(2, 2)
+    c2 = _lib.Context(0)
+    c2.set_samples(st.n_atoms, 1, _lib.SAMPLES_RADEMACHER, 1)
+    c2.set_operator(_lib.FORM_ORTHOGONAL, 0.0, 12.0)
+    c2.upload_bsr(_lib.SLOT_H, model.hamiltonian.build(THETA))
+    for ax in range(3):
+        c2.upload_bsr(_lib.SLOT_VX + ax, model.velocity[ax].build(THETA))
+    with pytest.raises(RuntimeError, match="tangent"):
+        c2.dc_spectrum_tangent(8)

@@ -139,3 +139,27 @@ def test_toy_bastin_fftcheb_equals_moment_reference():
         fft = kg.bastin_fftcheb(L, R, M, n_samples=ns, prefactor=3.0)
         assert np.max(np.abs(mom - fft)) < 1e-12 * max(1.0, np.max(np.abs(mom)))
         assert np.all(mom[0] == 0.0)  # the integral starts at the lowest node
+
+
+def test_linear_transport_model_velocity_basis():
+    """Every parameter's velocity factor B_{a,k} equals the oracle's bond
+    velocity V_a = i sum_e bond_a H_e of that parameter's edge entries
+    (commutator_velocity.hpp:7-43), and H_k its hopping and onsite entries;
+    H(theta), B_a(theta) are linear in theta."""
+    from transport_cases import THETA, graphene_transport
+
+    st, e, model, ec, oc = graphene_transport(3, 2)
+    n = st.n_atoms
+    for k in range(ec.shape[1]):
+        ek = np.zeros(ec.shape[1])
+        ek[k] = 1.0
+        V = kg.bond_velocity(n, 1, e.src, e.dst, e.vec, [np.array([[c]]) for c in ec[:, k]])
+        for a in range(3):
+            np.testing.assert_allclose(1j * model.velocity[a].build(ek).to_dense(), V[a], rtol=0, atol=1e-13)
+        Hd = np.diag(oc[:, k]).astype(float)
+        np.add.at(Hd, (e.src, e.dst), ec[:, k])
+        np.testing.assert_allclose(model.hamiltonian.build(ek).to_dense(), Hd, rtol=0, atol=1e-13)
+    H = model.hamiltonian.build(THETA).to_dense()
+    assert np.allclose(H, H.T)
+    Bx = model.velocity[0].build(THETA).to_dense()
+    assert np.allclose(Bx, -Bx.T)  # V = i B Hermitian
