@@ -56,6 +56,10 @@ def parse():
     p.add_argument("--kg-samples", type=int, default=16)
     p.add_argument("--kg-moments", type=int, default=512)
     p.add_argument("--kg-cpu-moments", type=int, default=128)
+    p.add_argument("--no-tblg", action="store_true", help="skip the differentiable twisted-bilayer transport leg")
+    p.add_argument("--tblg-reps", type=int, default=4, help="1.05-degree moire cells per edge (tBLG leg)")
+    p.add_argument("--tblg-samples", type=int, default=16)
+    p.add_argument("--tblg-moments", type=int, default=512)
     return p.parse_args()
 
 
@@ -319,6 +323,106 @@ def kg_leg(cells: int, R: int, M: int, cpu_M: int) -> dict:
                              "sample": f"one sample, {cpu_M} moments, scipy sparse oracle, {ct:.2f} s"}}
 
 
+def tblg_transport_model(reps: int):
+    """Twisted bilayer graphene at 1.05 degrees ((31, 32) commensurate cell,
+    11,908 atoms) tiled reps x reps, one p_z orbital per atom, and the two-
+    parameter hopping model H = theta_intra H_intra + theta_inter H_inter:
+    t(r) = Vpp_pi(r) (1 - n^2) + Vpp_sigma(r) n^2 with Vpp_pi = -2.7
+    exp(-(r - 1.42) / 0.319) and Vpp_sigma = 0.48 exp(-(r - 3.35) / 0.319) eV,
+    n the bond's z cosine, on every pair within 3.8 Angstrom; orthogonal
+    basis."""
+    import numpy as np
+
+    from rsatb_amd.autograd import linear_transport_model
+    from rsatb_amd.lattice import neighbor_edges, tile, twisted_bilayer_graphene
+
+    st = tile(twisted_bilayer_graphene(31, 32), (reps, reps, 1))
+    e = neighbor_edges(st, 3.8)
+    inter = st.atom_type[e.src] != st.atom_type[e.dst]
+    n2 = (e.vec[:, 2] / e.dist) ** 2
+    t_r = -2.7 * np.exp(-(e.dist - 1.42) / 0.319) * (1 - n2) + 0.48 * np.exp(-(e.dist - 3.35) / 0.319) * n2
+    ec = np.zeros((e.src.size, 2))
+    ec[~inter, 0] = t_r[~inter]
+    ec[inter, 1] = t_r[inter]
+    model = linear_transport_model(st.n_atoms, e, ec, np.zeros((st.n_atoms, 2)), ("intra", "inter"))
+    return st, e, model
+
+
+def tblg_leg(reps: int, R: int, M: int) -> dict:
+    """Differentiable Kubo-Greenwood on twisted bilayer graphene: the
+    Greenwood spectrum and its derivative with respect to the interlayer
+    hopping scale (rg_dc_spectrum_tangent), R Rademacher samples, M moments,
+    FP64, window from the Gershgorin bound. sigma(E_F = 0) and its derivative
+    are read at the node interpolation of E = 0; the derivative is checked at
+    scale against central differences of device spectra (h = 1e-4 and 2e-4,
+    Richardson-extrapolated)."""
+    import numpy as np
+
+    from rsatb_amd import _lib
+    from rsatb_amd.transport import interpolate_spectrum_component
+
+    t_setup = time.perf_counter()
+    st, e, model = tblg_transport_model(reps)
+    N = st.n_atoms
+    theta = np.array([1.0, 1.0])
+    H = model.hamiltonian.build(theta)
+    rows = np.repeat(np.arange(N), np.diff(H.row_ptr))
+    half = float(np.max(np.bincount(rows, weights=np.abs(H.values.reshape(-1)), minlength=N))) + 0.05
+    volume = float(abs(np.linalg.det(st.cell)))
+    pref = np.pi * 2.0 * 4.0 * N / volume * 2.4341348e6  # S/m with both spins, axis "abc"
+    e1 = np.array([0.0, 1.0])
+    setup_s = time.perf_counter() - t_setup
+
+    def bind(c, th):
+        c.upload_bsr(_lib.SLOT_H, model.hamiltonian.build(th))
+        for a in range(3):
+            c.upload_bsr(_lib.SLOT_VX + a, model.velocity[a].build(th))
+
+    with _lib.Context(0) as c:
+        c.set_samples(N, R, _lib.SAMPLES_RADEMACHER, 20240917)
+        c.set_operator(_lib.FORM_ORTHOGONAL, 0.0, half)
+        bind(c, theta)
+        c.upload_bsr(_lib.SLOT_DH, model.hamiltonian.build(e1))
+        for a in range(3):
+            c.upload_bsr(_lib.SLOT_DVX + a, model.velocity[a].build(e1))
+        c.dc_spectrum_tangent(16)  # warm-up
+        t0 = time.perf_counter()
+        sigma0, energy = c.dc_spectrum(M, 0, _lib.KERNEL_JACKSON, pref)
+        dt0 = time.perf_counter() - t0
+        c.set_profiling(True)
+        t0 = time.perf_counter()
+        sigma, dsigma, energy = c.dc_spectrum_tangent(M, 0, _lib.KERNEL_JACKSON, pref)
+        dt = time.perf_counter() - t0
+        ks = {name: c.kernel_stats(k) for k, name in _lib.KCLASS_NAMES.items()}
+        c.set_profiling(False)
+        def central(h):
+            bind(c, theta + h * e1)
+            sp, _ = c.dc_spectrum(M, 0, _lib.KERNEL_JACKSON, pref)
+            bind(c, theta - h * e1)
+            sm, _ = c.dc_spectrum(M, 0, _lib.KERNEL_JACKSON, pref)
+            return (sp - sm) / (2 * h)
+
+        # Richardson: the O(h^2) terms of two central differences cancel
+        fd = (4.0 * central(1e-4) - central(2e-4)) / 3.0
+    s_f = interpolate_spectrum_component(energy, sigma, 0.0)
+    ds_f = interpolate_spectrum_component(energy, dsigma, 0.0)
+    fd_f = interpolate_spectrum_component(energy, fd, 0.0)
+    diag = [0, 3]
+    return {"workload": f"twisted bilayer graphene 1.05 deg ((31,32) cell, 11908 atoms) tiled {reps}x{reps} "
+                        f"({N} atoms, p_z, {e.src.size} bonds within 3.8 A), orthogonal, R={R} Rademacher samples, "
+                        f"M={M} moments, Greenwood, d/d(interlayer hopping scale), FP64",
+            "ms_sigma": round(dt0 * 1e3, 1), "ms_sigma_and_tangent": round(dt * 1e3, 1),
+            "tangent_over_sigma": round(dt / dt0, 2), "host_setup_s": round(setup_s, 1),
+            "value": float(N) * R * M / dt, "unit": "orbital*vector*moment/s (spectrum + tangent)",
+            "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "count": v["count"]} for k, v in ks.items()
+                        if v["count"]},
+            "sigma_EF0_S_per_m": [float(s_f[i]) for i in diag],
+            "dsigma_dinter_EF0_S_per_m": [float(ds_f[i]) for i in diag],
+            "fd_rel_diff_EF0": float(np.max(np.abs(ds_f[diag] - fd_f[diag])) / np.max(np.abs(fd_f[diag]))),
+            "fd_rel_diff_spectrum": float(np.max(np.abs(dsigma - fd)) / np.max(np.abs(fd))),
+            "sigma_matches_plain_call": bool(np.max(np.abs(sigma - sigma0)) <= 1e-10 * np.max(np.abs(sigma0)))}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -391,6 +495,8 @@ def main() -> int:
         line["sk_assembly"] = sk_leg(args.sk_cells, 3, args.sk_cpu_edges)
     if not args.no_kg:
         line["kubo_greenwood"] = kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)
+    if not args.no_tblg:
+        line["tblg_transport"] = tblg_leg(args.tblg_reps, args.tblg_samples, args.tblg_moments)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(calc, args.cpu_moments, mu)
     print(json.dumps(line), flush=True)

@@ -5,44 +5,91 @@
 // the operator's sparsity pattern, dL/dA_ij += coef sum_c Re(u_i,c conj v_j,c).
 #include "rg_internal.h"
 
+#include <type_traits>
+
 namespace rg {
 namespace {
 
-__global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, int64_t nnzb,
-                                                    const int64_t *__restrict__ row_ptr,
+// G[e] += coef * sum_c Re(U[i, c] conj(V[j, c])) for every stored entry e of
+// the pattern. One group of LPR lanes per point row i (lanes over the sample
+// columns, c = q + t LPR): the row's U values stay in registers while the
+// group walks the row's entries, gathering each V row with coalesced loads
+// and summing the lane partials by xor shuffles. Entries are owned by their
+// point row, so no two groups write the same G element. For b = 1, four
+// entries' V rows are loaded before their reductions.
+template <bool CPLX, int LPR, int UPL, bool B1>
+__global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const int64_t *__restrict__ row_ptr,
                                                     const int32_t *__restrict__ col_ind,
-                                                    const double *__restrict__ U,
-                                                    const double *__restrict__ V, int R, int cplx,
-                                                    double coef, double *__restrict__ G) {
-    const int per = b * b;
-    const int64_t n = nnzb * per;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = t / per;
-        const int e = (int)(t - s * per);
-        const int cc = e / b, rr = e - cc * b; // column-major element (rr, cc)
-        int64_t lo = 0, hi = nbr - 1;          // block row of s
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (row_ptr[mid] <= s) lo = mid;
-            else hi = mid - 1;
+                                                    const double *__restrict__ U, const double *__restrict__ V,
+                                                    int R, double coef, double *__restrict__ G) {
+    using VT = std::conditional_t<CPLX, double2, double>;
+    constexpr int NG = 64 / LPR;
+    const VT *__restrict__ Uv = reinterpret_cast<const VT *>(U);
+    const VT *__restrict__ Vv = reinterpret_cast<const VT *>(V);
+    const int lane = threadIdx.x & 63;
+    const int q = lane % LPR;
+    const int64_t nrows = nbr * b;
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * NG;
+    auto dot = [](VT a, VT w) {
+        if constexpr (CPLX) return __builtin_fma(a.x, w.x, a.y * w.y);
+        else return a * w;
+    };
+    auto vzero = []() {
+        if constexpr (CPLX) return double2{0.0, 0.0};
+        else return 0.0;
+    };
+    auto reduce = [](double v) {
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, LPR);
+        return v;
+    };
+    for (int64_t i = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * NG + lane / LPR; i < nrows;
+         i += stride) {
+        const int64_t br = B1 ? i : i / b;
+        const int rr = B1 ? 0 : (int)(i - br * b);
+        VT u[UPL];
+#pragma unroll
+        for (int t = 0; t < UPL; ++t) {
+            const int c = q + t * LPR;
+            u[t] = c < R ? Uv[i * R + c] : vzero();
         }
-        const int64_t i = lo * b + rr;
-        const int64_t j = (int64_t)col_ind[s] * b + cc;
-        double acc = 0.0;
-        if (cplx) {
-            const double2 *u = reinterpret_cast<const double2 *>(U) + i * R;
-            const double2 *v = reinterpret_cast<const double2 *>(V) + j * R;
-            for (int c = 0; c < R; ++c) {
-                const double2 a = u[c], w = v[c];
-                acc = __builtin_fma(a.x, w.x, __builtin_fma(a.y, w.y, acc));
+        const int64_t s0 = row_ptr[br], s1 = row_ptr[br + 1];
+        int64_t s = s0;
+        if constexpr (B1) {
+            constexpr int K = 4;
+            for (; s + K <= s1; s += K) {
+                double acc[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int64_t j = col_ind[s + k];
+                    acc[k] = 0.0;
+#pragma unroll
+                    for (int t = 0; t < UPL; ++t) {
+                        const int c = q + t * LPR;
+                        if (c < R) acc[k] += dot(u[t], Vv[j * R + c]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] = reduce(acc[k]);
+                if (q == 0)
+#pragma unroll
+                    for (int k = 0; k < K; ++k) G[s + k] += coef * acc[k];
             }
-        } else {
-            const double *u = U + i * R;
-            const double *v = V + j * R;
-            for (int c = 0; c < R; ++c) acc = __builtin_fma(u[c], v[c], acc);
         }
-        G[t] += coef * acc;
+        for (; s < s1; ++s) {
+            const int64_t jb = (int64_t)col_ind[s] * b;
+            for (int cc = 0; cc < b; ++cc) {
+                double acc = 0.0;
+#pragma unroll
+                for (int t = 0; t < UPL; ++t) {
+                    const int c = q + t * LPR;
+                    if (c < R) acc += dot(u[t], Vv[(jb + cc) * R + c]);
+                }
+                acc = reduce(acc);
+                // column-major element (rr, cc) of block s
+                if (q == 0) G[s * b * b + (int64_t)cc * b + rr] += coef * acc;
+            }
+        }
     }
 }
 
@@ -68,12 +115,40 @@ int grid_for(int64_t n) {
 
 } // namespace
 
+template <bool CPLX, int LPR, int UPL>
+hipError_t go_grad(int64_t nbr, int b, const int64_t *row_ptr, const int32_t *col_ind, const double *U,
+                   const double *V, int R, double coef, double *G, hipStream_t st) {
+    const int64_t rows_per_wg = 4 * (64 / LPR);
+    int64_t g = (nbr * b + rows_per_wg - 1) / rows_per_wg;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    if (b == 1)
+        hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, true>), dim3((unsigned)g), dim3(256), 0, st, nbr, b,
+                           row_ptr, col_ind, U, V, R, coef, G);
+    else
+        hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, false>), dim3((unsigned)g), dim3(256), 0, st, nbr, b,
+                           row_ptr, col_ind, U, V, R, coef, G);
+    return hipGetLastError();
+}
+
+template <bool CPLX>
+hipError_t grad_by_r(int64_t nbr, int b, const int64_t *row_ptr, const int32_t *col_ind, const double *U,
+                     const double *V, int R, double coef, double *G, hipStream_t st) {
+    if (R <= 8) return go_grad<CPLX, 8, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+    if (R <= 16) return go_grad<CPLX, 16, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+    if (R <= 32) return go_grad<CPLX, 32, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+    if (R <= 64) return go_grad<CPLX, 64, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+    if (R <= 128) return go_grad<CPLX, 64, 2>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+    return go_grad<CPLX, 64, 4>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+}
+
 hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
                              const int32_t *col_ind, const double *U, const double *V, int R,
                              bool cplx, double coef, double *G, hipStream_t st) {
-    hipLaunchKernelGGL(k_grad_accum, dim3(grid_for(nnzb * b * b)), dim3(256), 0, st, nbr, b, nnzb,
-                       row_ptr, col_ind, U, V, R, cplx ? 1 : 0, coef, G);
-    return hipGetLastError();
+    (void)nnzb;
+    if (R < 1 || R > 256) return hipErrorInvalidValue;
+    return cplx ? grad_by_r<true>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st)
+                : grad_by_r<false>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
 }
 
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st) {

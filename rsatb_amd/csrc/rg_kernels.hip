@@ -163,7 +163,37 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
 
         const int64_t s0 = row_ptr[br];
         const int64_t s1 = row_ptr[br + 1];
-        for (int64_t s = s0; s < s1; ++s) {
+        int64_t s = s0;
+        if constexpr (BS == 1) {
+            // Scalar entries: the gather of each entry's X row waits on its
+            // column index, so U entries' indices and values are loaded
+            // first and their U row loads issued back to back (the
+            // accumulation order is unchanged). U = 4 for complex vectors
+            // keeps them at 8 waves per SIMD.
+            constexpr int U = CV ? 4 : 8;
+            for (; s + U <= s1; s += U) {
+                int64_t bc[U];
+                MT a[U];
+#pragma unroll
+                for (int t = 0; t < U; ++t) {
+                    bc[t] = col_ind[s + t];
+                    a[t] = vals[s + t];
+                }
+                VT xk[U][UPL];
+#pragma unroll
+                for (int t = 0; t < U; ++t)
+#pragma unroll
+                    for (int j = 0; j < UPL; ++j) {
+                        const int u = q * UPL + j;
+                        xk[t][j] = (u < R) ? X[bc[t] * R + u] : vzero<VT>();
+                    }
+#pragma unroll
+                for (int t = 0; t < U; ++t)
+#pragma unroll
+                    for (int j = 0; j < UPL; ++j) mac(acc[0][j], a[t], xk[t][j]);
+            }
+        }
+        for (; s < s1; ++s) {
             const int64_t bc = col_ind[s];
             const MT *__restrict__ blk = vals + s * (int64_t)(bs * bs);
             const VT *__restrict__ xb = X + bc * (int64_t)bs * R;

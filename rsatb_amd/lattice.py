@@ -87,6 +87,48 @@ def graphene_primitive_cell(a_cc: float = 1.42, vacuum: float = 20.0) -> Structu
     return Structure(pos, cell, (True, True, False))
 
 
+def twisted_bilayer_graphene(m: int = 31, n: int = 32, a_cc: float = 1.42, layer_gap: float = 3.35,
+                             vacuum: float = 20.0) -> Structure:
+    """Commensurate twisted bilayer graphene cell of index (m, n): superlattice
+    vectors T1 = m a1 + n a2 and T2 = -n a1 + (m + n) a2 of the bottom layer,
+    the top layer rotated by the angle phi that takes n a1 + m a2 onto T1
+    (cos phi = (m^2 + 4 m n + n^2) / (2 (m^2 + m n + n^2)); (31, 32) gives
+    1.05 degrees and 2 (m^2 + m n + n^2) = 5954 atoms per layer). Bottom
+    layer atoms first; atom_type 0 bottom, 1 top."""
+    a = np.sqrt(3.0) * a_cc
+    a1 = np.array([a, 0.0])
+    a2 = np.array([0.5 * a, 0.5 * np.sqrt(3.0) * a])
+    basis = [np.zeros(2), (a1 + a2) / 3.0]
+    T = np.stack([m * a1 + n * a2, -n * a1 + (m + n) * a2])
+    Tinv = np.linalg.inv(T)
+    u = n * a1 + m * a2
+    phi = np.arctan2(u[0] * T[0, 1] - u[1] * T[0, 0], u @ T[0])
+    rot = np.array([[np.cos(phi), -np.sin(phi)], [np.sin(phi), np.cos(phi)]])
+    n_layer = 2 * (m * m + m * n + n * n)
+    span = 2 * (m + n) + 2
+    p, q = np.meshgrid(np.arange(-span, span + 1), np.arange(-span, span + 1), indexing="ij")
+    lat = p.reshape(-1, 1) * a1 + q.reshape(-1, 1) * a2
+    layers = []
+    for R in (np.eye(2), rot):
+        pts = np.concatenate([(lat + b) @ R.T for b in basis])
+        f = pts @ Tinv
+        f = f - np.floor(f + 1e-9)  # wrap into [0, 1) with a tolerance at the edges
+        key = np.round(f * 1e6).astype(np.int64)
+        _, first = np.unique(key, axis=0, return_index=True)
+        f = f[np.sort(first)]
+        if f.shape[0] != n_layer:
+            raise RuntimeError(f"twisted cell ({m}, {n}): {f.shape[0]} atoms in a layer, expected {n_layer}")
+        layers.append(f @ T)
+    z0 = 0.5 * vacuum
+    pos = np.concatenate([np.column_stack([layers[0], np.full(n_layer, z0)]),
+                          np.column_stack([layers[1], np.full(n_layer, z0 + layer_gap)])])
+    cell = np.zeros((3, 3))
+    cell[:2, :2] = T
+    cell[2, 2] = vacuum + layer_gap
+    types = np.concatenate([np.zeros(n_layer, np.int32), np.ones(n_layer, np.int32)])
+    return Structure(pos, cell, (True, True, False), types)
+
+
 def diamond_cell(a: float = 3.567) -> Structure:
     """8-atom conventional diamond cell."""
     fcc = np.array([[0, 0, 0], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]])

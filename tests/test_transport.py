@@ -90,3 +90,33 @@ def test_qe_window_nodes():
         assert len(nodes) >= 1
     with pytest.raises(RuntimeError, match="outside the spectral window"):
         tr.fermi_window_nodes(M, c, a, 50.0, 300.0)
+
+
+def test_twisted_bilayer_graphene_cell():
+    """The (31, 32) commensurate cell: 5954 atoms per layer (2 (m^2 + m n + n^2)),
+    a hexagonal superlattice of side a / (2 sin(phi / 2)) with phi = 1.05
+    degrees (cos phi = 5953 / 5954), and both layers periodic in it: every
+    atom has exactly three in-layer neighbours at 1.42 A, the bond
+    directions of the top layer turned by phi against the bottom's."""
+    from rsatb_amd.lattice import neighbor_edges, twisted_bilayer_graphene
+
+    st = twisted_bilayer_graphene(31, 32)
+    assert st.n_atoms == 2 * 5954
+    assert np.count_nonzero(st.atom_type == 0) == 5954
+    phi = math.acos(5953 / 5954)
+    assert abs(math.degrees(phi) - 1.050) < 1e-3
+    a = math.sqrt(3) * 1.42
+    side = a / (2 * math.sin(phi / 2))
+    np.testing.assert_allclose(np.linalg.norm(st.cell[:2], axis=1), side, rtol=1e-12)
+    assert abs(st.cell[0] @ st.cell[1] - 0.5 * side * side) < 1e-8 * side * side
+    e = neighbor_edges(st, 1.6)
+    same = st.atom_type[e.src] == st.atom_type[e.dst]
+    assert np.all(same)
+    np.testing.assert_allclose(e.dist, 1.42, atol=1e-9)
+    np.testing.assert_array_equal(np.bincount(e.src, minlength=st.n_atoms), 3)
+    ang = np.arctan2(e.vec[:, 1], e.vec[:, 0]) % (math.pi / 3)  # bonds of both sublattices
+    bottom = np.unique(np.round(ang[st.atom_type[e.src] == 0], 9))
+    top = np.unique(np.round(ang[st.atom_type[e.src] == 1], 9))
+    assert bottom.size == 1 and top.size == 1
+    d = (top[0] - bottom[0]) % (math.pi / 3)
+    assert abs(min(d, math.pi / 3 - d) - phi) < 1e-9
