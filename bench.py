@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--kg-samples", type=int, default=16)
     p.add_argument("--kg-moments", type=int, default=512)
     p.add_argument("--kg-cpu-moments", type=int, default=128)
+    p.add_argument("--no-graphene-scale", action="store_true", help="skip the b=1 graphene scale-up leg")
+    p.add_argument("--graphene-cells", type=int, default=512, help="rectangular cells per edge (b=1 scale-up leg)")
     p.add_argument("--no-tblg", action="store_true", help="skip the differentiable twisted-bilayer transport leg")
     p.add_argument("--tblg-reps", type=int, default=4, help="1.05-degree moire cells per edge (tBLG leg)")
     p.add_argument("--tblg-samples", type=int, default=16)
@@ -323,6 +325,52 @@ def kg_leg(cells: int, R: int, M: int, cpu_M: int) -> dict:
                              "sample": f"one sample, {cpu_M} moments, scipy sparse oracle, {ct:.2f} s"}}
 
 
+def graphene_scale_leg(cells: int, R: int = 16, M: int = 1024) -> dict:
+    """The plain-DOS graphene operator (p_z, nearest-neighbour t = -2.7 eV,
+    onsite 0 stored, window b = 0, a = 8.1 / 0.99) at cells x cells
+    rectangular cells, R Rademacher samples, M doubled moments, FP64 and
+    FP32: the b = 1 fused step's achieved HBM bandwidth on a working set that
+    does not fit the caches (at 512 x 512 cells: 1,048,576 orbitals, 134 MB
+    per FP64 vector)."""
+    import numpy as np
+
+    from rsatb_amd import _lib
+    from rsatb_amd.lattice import graphene_rect_supercell, neighbor_edges
+    from rsatb_amd.sktb import Radial, SkParameters, SktbModel
+
+    h = SkParameters().set_onsite("C", "pz", 0.0)
+    h.set_pair_channel("C", "C", "pp", "pi", Radial.spline([0.0, 1.6], [-2.7, -2.7], 1.6))
+    model = SktbModel({"C": ["pz"]}, h)
+    st = graphene_rect_supercell(cells, cells)
+    e = neighbor_edges(st, model.cutoff)
+    N = st.n_atoms
+    out = {"workload": f"graphene {cells}x{cells} rectangular cells ({N} atoms, p_z), nearest-neighbour t=-2.7 eV, "
+                       f"window b=0 a=8.1/0.99, R={R} Rademacher samples, M={M} moments (doubled)"}
+    with _lib.Context(0) as c:
+        c.sk_assemble(_lib.SLOT_H, model.arrays("H"), st.atom_type, st.positions, st.cell,
+                      np.stack([e.src, e.dst], axis=1), e.shift)
+        c.set_operator(_lib.FORM_ORTHOGONAL, 0.0, 8.1 / 0.99)
+        c.set_samples(N, R, _lib.SAMPLES_RADEMACHER, 20240917)
+        mus = {}
+        for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
+            c.kpm_moments(16, prec, _lib.DOUBLING_ON)  # warm-up
+            c.set_profiling(True)
+            t0 = time.perf_counter()
+            mus[name] = c.kpm_moments(M, prec, _lib.DOUBLING_ON)
+            dt = time.perf_counter() - t0
+            st_ = c.kernel_stats(_lib.KCLASS_SPMM_STEP)
+            c.set_profiling(False)
+            gbs = st_["bytes"] / (st_["avg_ms"] * 1e-3) / 1e9
+            out[name] = {"ms": round(dt * 1e3, 2), "value": float(N) * R * M / dt,
+                         "unit": "orbital*vector*moment/s", "step_avg_us": round(st_["avg_ms"] * 1e3, 2),
+                         "step_algorithmic_MB": round(st_["bytes"] / 1e6, 2), "step_GBs": round(gbs, 1),
+                         "step_hbm_frac": round(gbs / 8000.0, 3)}
+    out["fp32_speedup"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)
+    out["max_abs_dmu_fp32"] = float(np.max(np.abs(mus["fp32"] - mus["fp64"])))
+    out["mu0"] = float(mus["fp64"][0].real)
+    return out
+
+
 def tblg_transport_model(reps: int):
     """Twisted bilayer graphene at 1.05 degrees ((31, 32) commensurate cell,
     11,908 atoms) tiled reps x reps, one p_z orbital per atom, and the two-
@@ -495,6 +543,8 @@ def main() -> int:
         line["sk_assembly"] = sk_leg(args.sk_cells, 3, args.sk_cpu_edges)
     if not args.no_kg:
         line["kubo_greenwood"] = kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)
+    if not args.no_graphene_scale:
+        line["graphene_scale"] = graphene_scale_leg(args.graphene_cells)
     if not args.no_tblg:
         line["tblg_transport"] = tblg_leg(args.tblg_reps, args.tblg_samples, args.tblg_moments)
     if not args.no_cpu_baseline:

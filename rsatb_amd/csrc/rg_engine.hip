@@ -623,6 +623,7 @@ int rg_create(int device_id, rg_context **out) {
     ctx->device = device_id;
     if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
+    if (const char *v = std::getenv("RG_B1_WIDE")) set_b1_wide(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD32")) set_mfma_pd32(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD64")) set_mfma_pd64(std::atoi(v));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -63,6 +63,7 @@ int spmm_max_grid();
 // XCD-contiguous row ranges in the MFMA SpMM (default on).
 void set_xcd_map(int on);
 void set_mfma_ntcap(int nt);
+void set_b1_wide(int mode);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
 

@@ -37,6 +37,9 @@ constexpr int kMaxGrid = 2048;
 int g_xcd_map = 1;
 // Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
 int g_mfma_ntcap = 16;
+// b = 1 SpMM over real vectors with 2 columns per lane: 1 always, 0 never,
+// 2 by precision and row length.
+int g_b1_wide = 2;
 // Register prefetch depths of the MFMA SpMM: FP64 plain products 3 (2 when
 // set to 2; steps always 2: their epilogue needs the registers), FP32 3 or 4.
 // Measured at the diamond 8^3 sandwich, R = 64 complex: FP64 plain 105.6 us
@@ -301,6 +304,18 @@ hipError_t go(const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, 
 template <typename S, bool CM, bool CV, int BS>
 hipError_t by_layout(const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, int grid,
                      hipStream_t st) {
+    if constexpr (BS == 1 && !CV) {
+        // Scalar entries, real vectors: half the lanes per row and two
+        // columns per lane, twice the rows (and independent gathers) per
+        // wave. Measured on graphene p_z with R = 16: FP32 step 103 -> 76 us
+        // at 1,048,576 rows of 4 entries, FP64 86 -> 75 us at 190,528 rows of
+        // 22 entries, but FP64 163 -> 193 us at 4 entries a row; complex
+        // vectors (R = 32) slower. Automatic: FP32, or rows of >= 8 entries.
+        const bool wide = g_b1_wide == 1 ||
+                          (g_b1_wide == 2 && (sizeof(S) == 4 || A.nnzb >= 8 * A.nbr));
+        if (wide && R > 8 && R <= 16) return go<S, CM, CV, BS, 8, 2>(A, X, Y, R, e, grid, st);
+        if (wide && R > 16 && R <= 32) return go<S, CM, CV, BS, 16, 2>(A, X, Y, R, e, grid, st);
+    }
     switch (lpr_for(R)) {
     case 8: return go<S, CM, CV, BS, 8, 1>(A, X, Y, R, e, grid, st);
     case 16: return go<S, CM, CV, BS, 16, 1>(A, X, Y, R, e, grid, st);
@@ -881,6 +896,7 @@ int spmm_max_grid() { return kMaxGrid; }
 
 void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
+void set_b1_wide(int mode) { g_b1_wide = mode == 0 || mode == 1 ? mode : 2; }
 void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
 void set_mfma_pd64(int p) { g_mfma_pd64 = p == 2 ? 2 : 3; }
 
