@@ -171,9 +171,11 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
             // Scalar entries: the gather of each entry's X row waits on its
             // column index, so U entries' indices and values are loaded
             // first and their U row loads issued back to back (the
-            // accumulation order is unchanged). U = 4 for complex vectors
-            // keeps them at 8 waves per SIMD.
-            constexpr int U = CV ? 4 : 8;
+            // accumulation order is unchanged). U = 4 measured best: 4-entry
+            // graphene rows take it whole (FP64 step 161 -> 137 us at
+            // 1,048,576 rows), 22-entry rows gain over U = 8 (74 -> 67 us),
+            // and complex vectors stay at 8 waves per SIMD.
+            constexpr int U = 4;
             for (; s + U <= s1; s += U) {
                 int64_t bc[U];
                 MT a[U];
