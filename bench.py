@@ -404,7 +404,8 @@ def tblg_leg(reps: int, R: int, M: int) -> dict:
     """Differentiable Kubo-Greenwood on twisted bilayer graphene: the
     Greenwood spectrum and its derivative with respect to the interlayer
     hopping scale (rg_dc_spectrum_tangent), R Rademacher samples, M moments,
-    FP64, window from the Gershgorin bound. sigma(E_F = 0) and its derivative
+    FP64, window from the Gershgorin bound; the spectrum also in the FP32
+    form (FP32 recurrence, FP64 moment GEMMs). sigma(E_F = 0) and its derivative
     are read at the node interpolation of E = 0; the derivative is checked at
     scale against central differences of device spectra (h = 1e-4 and 2e-4,
     Richardson-extrapolated)."""
@@ -441,6 +442,10 @@ def tblg_leg(reps: int, R: int, M: int) -> dict:
         t0 = time.perf_counter()
         sigma0, energy = c.dc_spectrum(M, 0, _lib.KERNEL_JACKSON, pref)
         dt0 = time.perf_counter() - t0
+        c.dc_spectrum(16, 0, _lib.KERNEL_JACKSON, pref, precision=_lib.PREC_FP32)  # warm-up
+        t0 = time.perf_counter()
+        sigma32, _ = c.dc_spectrum(M, 0, _lib.KERNEL_JACKSON, pref, precision=_lib.PREC_FP32)
+        dt32 = time.perf_counter() - t0
         t0 = time.perf_counter()
         sigma, dsigma, energy = c.dc_spectrum_tangent(M, 0, _lib.KERNEL_JACKSON, pref)
         dt = time.perf_counter() - t0
@@ -465,6 +470,11 @@ def tblg_leg(reps: int, R: int, M: int) -> dict:
                         f"({N} atoms, p_z, {e.src.size} bonds within 3.8 A), orthogonal, R={R} Rademacher samples, "
                         f"M={M} moments, Greenwood, d/d(interlayer hopping scale), FP64",
             "ms_sigma": round(dt0 * 1e3, 1), "ms_sigma_and_tangent": round(dt * 1e3, 1),
+            "ms_sigma_fp32": round(dt32 * 1e3, 1), "fp32_speedup": round(dt0 / dt32, 3),
+            "fp32_max_diff_rel": float(np.max(np.abs(sigma32 - sigma0)) / np.max(np.abs(sigma0))),
+            "fp32_EF0_rel_diff": float(np.max(np.abs(interpolate_spectrum_component(energy, sigma32, 0.0)[[0, 3]]
+                                                     - interpolate_spectrum_component(energy, sigma0, 0.0)[[0, 3]]))
+                                       / np.max(np.abs(interpolate_spectrum_component(energy, sigma0, 0.0)[[0, 3]]))),
             "tangent_over_sigma": round(dt / dt0, 2), "host_setup_s": round(setup_s, 1),
             "value": float(N) * R * M / dt, "unit": "orbital*vector*moment/s (spectrum + tangent)",
             "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "count": v["count"]} for k, v in ks.items()

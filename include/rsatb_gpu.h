@@ -282,6 +282,15 @@ RG_API int rg_kubo_greenwood(rg_context *ctx, int M, int kernel, double prefacto
 RG_API int rg_dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor,
                           double *sigma_out, double *energy_out, double *mu_out);
 
+/* rg_dc_spectrum with a precision (RG_PREC_FP64 or RG_PREC_FP32). FP32: the
+ * recurrence and the velocity products run on FP32 vectors and matrices and
+ * the stored sequences are FP32 (half the memory); the moment GEMMs widen
+ * K-chunks of them to FP64 and accumulate in FP64. The reference's transport
+ * path is FP64 only (dc_spectrum_fftcheb, fftcheb_transport.hpp:543-653);
+ * this is the precision form of that spectrum. */
+RG_API int rg_dc_spectrum_ex(rg_context *ctx, int M, int formulation, int kernel, double prefactor, int precision,
+                             double *sigma_out, double *energy_out, double *mu_out);
+
 /* rg_dc_spectrum and its directional derivative: dsigma_out[k * 6 + comp] =
  * d sigma / d e along H -> H + e dH, B_a -> B_a + e dB_a (RG_SLOT_DH,
  * RG_SLOT_DVX..), with the window of rg_set_operator held fixed. The

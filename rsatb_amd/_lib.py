@@ -78,6 +78,7 @@ SIGNATURES = {
     "rg_kubo_greenwood": (_i, [_vp, _i, _i, _d, _pd, _pd, _pd]),
     "rg_dc_spectrum": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
     "rg_dc_spectrum_tangent": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
+    "rg_dc_spectrum_ex": (_i, [_vp, _i, _i, _i, _d, _i, _pd, _pd, _pd]),
     "rg_isqrt_patches": (_i, [_vp, _d, _i, C.POINTER(_i), _pd]),
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_set_checkpointing": (_i, [_vp, _i]),
@@ -244,16 +245,18 @@ class Context:
         return (sigma, energy, mu) if with_moments else (sigma, energy)
 
     def dc_spectrum(self, n_moments: int, formulation: int = 0, kernel: int = KERNEL_JACKSON,
-                    prefactor: float = 1.0, with_moments: bool = False):
+                    prefactor: float = 1.0, with_moments: bool = False, precision: int = PREC_FP64):
         """(sigma [M, 6] in (xx, xy, xz, yy, yz, zz), energies [M] [, mu
-        [pairs, M, M]]); formulation 0 Greenwood, 1 Bastin."""
+        [pairs, M, M]]); formulation 0 Greenwood, 1 Bastin; precision
+        PREC_FP32 runs the recurrence in FP32 with FP64 moment accumulation."""
         M = int(n_moments)
         sigma = np.zeros((M, 6))
         energy = np.zeros(M)
         mu = np.zeros((6 if formulation == 1 else 3, M, M), dtype=np.complex128) if with_moments else None
-        rc = self._lib.rg_dc_spectrum(self._h, M, int(formulation), int(kernel), float(prefactor), _dptr(sigma),
-                                      _dptr(energy), _dptr(mu.view(np.float64)) if with_moments else None)
-        _check(self._h, rc, "rg_dc_spectrum")
+        rc = self._lib.rg_dc_spectrum_ex(self._h, M, int(formulation), int(kernel), float(prefactor), int(precision),
+                                         _dptr(sigma), _dptr(energy),
+                                         _dptr(mu.view(np.float64)) if with_moments else None)
+        _check(self._h, rc, "rg_dc_spectrum_ex")
         return (sigma, energy, mu) if with_moments else (sigma, energy)
 
     def dc_spectrum_tangent(self, n_moments: int, formulation: int = 0, kernel: int = KERNEL_JACKSON,

@@ -106,6 +106,17 @@ __global__ __launch_bounds__(256) void k_axpby(double *__restrict__ y, const dou
         y[t] = alpha * x[t] + beta * y[t];
 }
 
+// out[c * rows + r] = in[c * ld_in + r] widened to double, r < rows, c < cols
+__global__ __launch_bounds__(256) void k_widen_cols(const float *__restrict__ in, int64_t ld_in,
+                                                    double *__restrict__ out, int64_t rows, int cols) {
+    const int64_t n = rows * cols;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = t / rows, r = t - c * rows;
+        out[t] = (double)in[c * ld_in + r];
+    }
+}
+
 int grid_for(int64_t n) {
     int64_t g = (n + 255) / 256;
     if (g > 16384) g = 16384;
@@ -153,6 +164,11 @@ hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *ro
 
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(256), 0, st, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_cols(const float *in, int64_t ld_in, double *out, int64_t rows, int cols, hipStream_t st) {
+    hipLaunchKernelGGL(k_widen_cols, dim3(grid_for(rows * cols)), dim3(256), 0, st, in, ld_in, out, rows, cols);
     return hipGetLastError();
 }
 

@@ -103,6 +103,7 @@ struct rg_context {
     DevBuf sk_ints, sk_dbls, sk_type, sk_pos, sk_ei, sk_sh, sk_eblk, sk_flags, sk_sp, sk_src, sk_ph, sk_row;
     // Kubo-Greenwood: stored left/right sequences and the moment matrices
     DevBuf kg_store, kg_mu, kg_work;
+    size_t kg_widen_bytes = (size_t)8 << 30; // FP32 transport: FP64 chunk panels (RG_KG_WIDEN_BYTES)
     rocblas_handle blas = nullptr;
 
     std::atomic<int> cancel{0};
@@ -340,18 +341,25 @@ int check_operator(rg_context *ctx) {
     return RG_OK;
 }
 
+// FP32 copy of one slot's values (kept until the slot is written again).
+int ensure_fp32_slot(rg_context *ctx, int slot) {
+    Slot &s = ctx->slots[slot];
+    if (s.vals32_valid) return RG_OK;
+    const int64_t n = s.nnzb * s.b * s.b * (s.is_complex ? 2 : 1);
+    RG_HIP(ctx, s.vals32.ensure((size_t)n * 4), "hipMalloc fp32 values");
+    RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.vals64.p), static_cast<float *>(s.vals32.p), n,
+                                  ctx->stream),
+           "fp32 conversion");
+    s.vals32_valid = true;
+    return RG_OK;
+}
+
 // FP32 copies of the matrices the form uses.
 int ensure_fp32(rg_context *ctx) {
     const int slots[3] = {RG_SLOT_H, RG_SLOT_Z, RG_SLOT_ZH};
     for (int i = 0; i < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++i) {
-        Slot &s = ctx->slots[slots[i]];
-        if (s.vals32_valid) continue;
-        const int64_t n = s.nnzb * s.b * s.b * (s.is_complex ? 2 : 1);
-        RG_HIP(ctx, s.vals32.ensure((size_t)n * 4), "hipMalloc fp32 values");
-        RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.vals64.p),
-                                      static_cast<float *>(s.vals32.p), n, ctx->stream),
-               "fp32 conversion");
-        s.vals32_valid = true;
+        const int rc = ensure_fp32_slot(ctx, slots[i]);
+        if (rc) return rc;
     }
     return RG_OK;
 }
@@ -629,6 +637,10 @@ int rg_create(int device_id, rg_context **out) {
     if (const char *v = std::getenv("RG_B1_WIDE")) set_b1_wide(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD32")) set_mfma_pd32(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD64")) set_mfma_pd64(std::atoi(v));
+    if (const char *v = std::getenv("RG_KG_WIDEN_BYTES")) {
+        const long long n = std::atoll(v);
+        if (n > 0) ctx->kg_widen_bytes = (size_t)n;
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RG_ERR_HIP;
@@ -1258,9 +1270,13 @@ namespace {
 // diagonal pairs; Bastin (1) all six from mu^{ab}, a <= b. mu_out: [pairs][M][M]
 // complex, pairs 3 (Greenwood: xx, yy, zz) or 6 (Bastin, component order).
 int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor, double *sigma_out,
-                double *energy_out, double *mu_out, double *dsigma_out = nullptr) {
+                double *energy_out, double *mu_out, double *dsigma_out = nullptr, int precision = RG_PREC_FP64) {
     if (!ctx || !sigma_out) return RG_ERR_INVALID;
     const bool tangent = dsigma_out != nullptr;
+    if (precision != RG_PREC_FP64 && precision != RG_PREC_FP32)
+        return fail(ctx, RG_ERR_INVALID, "precision: RG_PREC_FP64 or RG_PREC_FP32");
+    const bool fp32 = precision == RG_PREC_FP32;
+    if (fp32 && tangent) return fail(ctx, RG_ERR_UNSUPPORTED, "the tangent spectrum runs in FP64 only");
     if (formulation != 0 && formulation != 1) return fail(ctx, RG_ERR_INVALID, "formulation: 0 Greenwood, 1 Bastin");
     int rc = check_operator(ctx);
     if (rc) return rc;
@@ -1292,6 +1308,14 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     const int R = ctx->R;
     VecMode vm;
     vm.complex_vec = vec_complex(ctx);
+    vm.fp32 = fp32;
+    if (fp32) {
+        const int need[4] = {RG_SLOT_H, RG_SLOT_VX, RG_SLOT_VY, RG_SLOT_VZ};
+        for (int q : need) {
+            rc = ensure_fp32_slot(ctx, q);
+            if (rc) return rc;
+        }
+    }
     const size_t eb = vec_elem_bytes(vm), K = (size_t)rows * R, nb = K * eb;
     for (int i = 0; i < 3; ++i) RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
     RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
@@ -1464,7 +1488,53 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
     double *dmud = Qz + 3 * (size_t)np * zmat; // tangent moments (mu_real doubles)
     const double s2 = 1.0 / (4.0 * ha * ha);
     // out[pi] (+)= s^2 Lseq(a)^H Rseq(b) for every pair
+    // FP32 sequences: K is cut into chunks of Kc rows; each chunk of the M
+    // left and M right vectors is widened to FP64 (k_widen_cols) and the
+    // GEMMs accumulate over the chunks in FP64 (beta = 1 after the first).
+    // Chunks stay long (8 GiB of panels by default): an M x M product over a
+    // short K has too few output tiles to fill the GPU (1 GiB chunks at M =
+    // 512 made the FP32 spectrum 5x slower than FP64).
+    const int cw = vm.complex_vec ? 2 : 1; // floats per element
+    size_t Kc = K;
+    if (fp32) {
+        const size_t budget = ctx->kg_widen_bytes; // bytes of the two FP64 chunk panels
+        Kc = std::max<size_t>(1, std::min(K, budget / (2 * (size_t)M * 8 * cw)));
+        RG_HIP(ctx, ctx->kg_work.ensure(2 * (size_t)M * Kc * 8 * cw), "hipMalloc widening panels");
+    }
     auto moment_gemms = [&](auto Lseq, auto Rseq, double *out, double beta_in) -> int {
+    if (fp32) {
+        double *Lw = static_cast<double *>(ctx->kg_work.p);
+        double *Rw = Lw + (size_t)M * Kc * cw;
+        for (int pi = 0; pi < np; ++pi) {
+            const int a = comp_a[pairs[pi]], b = comp_b[pairs[pi]];
+            for (size_t k0 = 0; k0 < K; k0 += Kc) {
+                const size_t kc = std::min(Kc, K - k0);
+                RG_HIP(ctx, launch_widen_cols(static_cast<const float *>(Lseq(a, 0)) + k0 * cw, (int64_t)K * cw,
+                                              Lw, (int64_t)kc * cw, M, ctx->stream),
+                       "widen launch");
+                RG_HIP(ctx, launch_widen_cols(static_cast<const float *>(Rseq(b, 0)) + k0 * cw, (int64_t)K * cw,
+                                              Rw, (int64_t)kc * cw, M, ctx->stream),
+                       "widen launch");
+                const double bk = k0 == 0 ? beta_in : 1.0;
+                rocblas_status st;
+                if (vm.complex_vec) {
+                    const rocblas_double_complex alpha{s2, 0.0}, beta{bk, 0.0};
+                    st = rocblas_zgemm(ctx->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, M,
+                                       M, (rocblas_int)kc, &alpha, reinterpret_cast<rocblas_double_complex *>(Lw),
+                                       (rocblas_int)kc, reinterpret_cast<rocblas_double_complex *>(Rw),
+                                       (rocblas_int)kc, &beta,
+                                       reinterpret_cast<rocblas_double_complex *>(out + pi * zmat), M);
+                } else {
+                    const double alpha = s2;
+                    st = rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, M, M,
+                                       (rocblas_int)kc, &alpha, Lw, (rocblas_int)kc, Rw, (rocblas_int)kc, &bk,
+                                       out + pi * mm, M);
+                }
+                if (st != rocblas_status_success) return fail(ctx, RG_ERR_HIP, "moment GEMM failed");
+            }
+        }
+        return RG_OK;
+    }
     for (int pi = 0; pi < np; ++pi) {
         const int a = comp_a[pairs[pi]], b = comp_b[pairs[pi]];
         rocblas_status st;
@@ -1629,6 +1699,11 @@ extern "C" {
 int rg_dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double prefactor, double *sigma_out,
                    double *energy_out, double *mu_out) {
     return dc_spectrum(ctx, M, formulation, kernel, prefactor, sigma_out, energy_out, mu_out);
+}
+
+int rg_dc_spectrum_ex(rg_context *ctx, int M, int formulation, int kernel, double prefactor, int precision,
+                      double *sigma_out, double *energy_out, double *mu_out) {
+    return dc_spectrum(ctx, M, formulation, kernel, prefactor, sigma_out, energy_out, mu_out, nullptr, precision);
 }
 
 int rg_dc_spectrum_tangent(rg_context *ctx, int M, int formulation, int kernel, double prefactor,

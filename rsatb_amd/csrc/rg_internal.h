@@ -135,6 +135,8 @@ hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *ro
                              bool cplx, double coef, double *G, hipStream_t st);
 // y = 0 over n doubles
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st);
+// out[c * rows + r] = (double) in[c * ld_in + r] for r < rows, c < cols
+hipError_t launch_widen_cols(const float *in, int64_t ld_in, double *out, int64_t rows, int cols, hipStream_t st);
 // y = alpha x + beta y over n doubles
 hipError_t launch_axpby(double *y, const double *x, double alpha, double beta, int64_t n, hipStream_t st);
 

@@ -334,6 +334,8 @@ class RSATB:
         emin, emax = self.energy_bounds()
         st, e = self.structure, self.edges
         n_rows = st.n_atoms * self.model.norb
+        # the precision form (SetupOptions.precision "fp32"): FP32 recurrence, FP64 moments
+        prec = _lib.PREC_FP32 if self.options.precision == "fp32" else _lib.PREC_FP64
         # pi g_s (4 Greenwood, 8 Bastin) N / volume (observables.hpp:722-733)
         prefactor = np.pi * 2.0 * (4.0 if formulation == "greenwood" else 8.0) * n_rows / self.transport_volume(axis)
         sigma = np.zeros((len(self._kstates), n_moments, 6))
@@ -344,7 +346,8 @@ class RSATB:
             self._bind_kpoint(k_idx)
             self._ctx.sk_velocity(self.model.arrays("H"), st.atom_type, st.positions, st.cell,
                                   np.stack([e.src, e.dst], axis=1), e.shift)
-            sigma[k_idx], energy = self._ctx.dc_spectrum(n_moments, forms[formulation], kernels[kernel], prefactor)
+            sigma[k_idx], energy = self._ctx.dc_spectrum(n_moments, forms[formulation], kernels[kernel], prefactor,
+                                                         precision=prec)
         # e^2 / (hbar Angstrom) in S/m (rsatb/driver.py:393)
         return sigma * 2.4341348 * 1e6, energy
 

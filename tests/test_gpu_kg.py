@@ -272,3 +272,37 @@ def test_dc_tangent_needs_tangent_slots():
         c2.upload_bsr(_lib.SLOT_VX + ax, model.velocity[ax].build(THETA))
     with pytest.raises(RuntimeError, match="tangent"):
         c2.dc_spectrum_tangent(8)
+
+
+@pytest.mark.parametrize("formulation", [0, 1])
+@pytest.mark.parametrize("kind", [_lib.SAMPLES_RADEMACHER, _lib.SAMPLES_PHASE])
+def test_fp32_transport_against_fp64(formulation, kind, monkeypatch):
+    """rg_dc_spectrum_ex with RG_PREC_FP32 (FP32 recurrence and velocity
+    products, FP32 stored sequences, FP64 moment GEMMs over widened
+    K-chunks) against the FP64 spectrum of the same inputs: within 1e-5 of
+    the largest FP64 value. With a small widening budget (many K-chunks,
+    RG_KG_WIDEN_BYTES) the FP32 spectrum equals the one-chunk FP32 spectrum
+    to 1e-12: the chunked FP64 accumulation only reorders the sums."""
+    spectra = {}
+    for budget in (None, "4096"):
+        if budget is None:
+            monkeypatch.delenv("RG_KG_WIDEN_BYTES", raising=False)
+        else:
+            monkeypatch.setenv("RG_KG_WIDEN_BYTES", budget)
+        c = _lib.Context(0)
+        try:
+            H, V = setup_si(c)
+            e = np.linalg.eigvalsh(H)
+            c.set_operator(_lib.FORM_ORTHOGONAL, 0.5 * (e[0] + e[-1]), 0.55 * (e[-1] - e[0]))
+            c.set_samples(H.shape[0], 3, kind, 5)
+            print("case: Si KG fp32 vs fp64, formulation", formulation, "kind", kind, "budget", budget, flush=True)
+            s64, _ = c.dc_spectrum(40, formulation, prefactor=1.0)
+            s32, en = c.dc_spectrum(40, formulation, prefactor=1.0, precision=_lib.PREC_FP32)
+            spectra[budget] = s32
+        finally:
+            c.close()
+        scale = np.max(np.abs(s64))
+        assert np.max(np.abs(s32 - s64)) <= 1e-5 * scale
+        assert np.max(np.abs(s32 - s64)) > 0.0  # the FP32 path ran
+    scale = np.max(np.abs(spectra[None]))
+    assert np.max(np.abs(spectra["4096"] - spectra[None])) <= 1e-12 * scale
