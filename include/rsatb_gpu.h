@@ -252,14 +252,20 @@ RG_API int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *table, i
  *      rsatb/calc/fftcheb_transport.hpp:543-623; velocity of an orthogonal basis,
  *      rsatb/operator/commutator_velocity.hpp:7-43) ---- */
 
-/* The commutator velocity V_a = i B_a of an orthogonal SK model at Gamma:
+/* The commutator velocity V_a = i B_a of an orthogonal SK model:
  * B_a = sum over edges of (pos_j + shift . cell - pos_i)_a times the edge's
  * SK block (AccumulateBondMoment with the displacement weight and factor i),
- * into RG_SLOT_VX .. RG_SLOT_VZ (real, H's stored pattern). Same arguments as
- * rg_sk_assemble with the Hamiltonian table. */
+ * into RG_SLOT_VX .. RG_SLOT_VZ on H's stored pattern. At kpoint (fractional,
+ * NULL or zero: Gamma, real factors) each edge also carries the Bloch phase
+ * e^{2 pi i k . shift} that rg_sk_assemble gives H(k), so B_a(k) is complex
+ * anti-Hermitian and V_a(k) the velocity in H(k)'s gauge (the reference's
+ * atomic-gauge V and H(k), commutator_velocity.hpp:24-31, are the same pair
+ * conjugated by diag(e^{i k . r})). Same arguments as rg_sk_assemble with the
+ * Hamiltonian table. */
 RG_API int rg_sk_velocity(rg_context *ctx, const rg_sk_table *table, int64_t n_atoms,
                           const int32_t *atom_type, const double *pos, const double *cell,
-                          int64_t n_edges, const int32_t *edge_index, const int32_t *edge_shift);
+                          int64_t n_edges, const int32_t *edge_index, const int32_t *edge_shift,
+                          const double *kpoint);
 
 /* Greenwood conductivity spectrum of the orthogonal form (H in RG_SLOT_H,
  * velocity factors in RG_SLOT_VX.., window of rg_set_operator, samples of

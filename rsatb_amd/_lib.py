@@ -74,7 +74,7 @@ SIGNATURES = {
     "rg_download_bsr": (_i, [_vp, _i, _pi64, _pi32, _pd]),
     "rg_sk_assemble": (_i, [_vp, _i, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32, _pd,
                             _pi64]),
-    "rg_sk_velocity": (_i, [_vp, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32]),
+    "rg_sk_velocity": (_i, [_vp, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32, _pd]),
     "rg_kubo_greenwood": (_i, [_vp, _i, _i, _d, _pd, _pd, _pd]),
     "rg_dc_spectrum": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
     "rg_dc_spectrum_tangent": (_i, [_vp, _i, _i, _i, _d, _pd, _pd, _pd]),
@@ -226,10 +226,11 @@ class Context:
         flattened (src, dst) pairs). Returns the stored block count."""
         return self._sk_call(False, slot, table_arrays, atom_type, pos, cell, edge_index, edge_shift, kpoint)
 
-    def sk_velocity(self, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift) -> None:
+    def sk_velocity(self, table_arrays: dict, atom_type, pos, cell, edge_index, edge_shift, kpoint=None) -> None:
         """Bond-moment velocity factors B_x, B_y, B_z (V = i B) of an
-        orthogonal SK model at Gamma into SLOT_VX .. SLOT_VZ."""
-        self._sk_call(True, SLOT_VX, table_arrays, atom_type, pos, cell, edge_index, edge_shift, None)
+        orthogonal SK model into SLOT_VX .. SLOT_VZ, at Gamma (real) or at a
+        fractional kpoint in the gauge of sk_assemble's H(k) (complex)."""
+        self._sk_call(True, SLOT_VX, table_arrays, atom_type, pos, cell, edge_index, edge_shift, kpoint)
 
     def kubo_greenwood(self, n_moments: int, kernel: int = KERNEL_JACKSON, prefactor: float = 1.0,
                        with_moments: bool = False):
@@ -299,13 +300,13 @@ class Context:
             raise ValueError("edge_index needs (src, dst) pairs and edge_shift three integers per edge")
         if pos.shape[0] != types.size:
             raise ValueError("pos and atom_type describe different atom counts")
+        kp = None if kpoint is None else np.ascontiguousarray(kpoint, dtype=np.float64).reshape(3)
         if velocity:
             rc = self._lib.rg_sk_velocity(self._h, C.byref(t), int(types.size), types.ctypes.data_as(_pi32),
                                           _dptr(pos), _dptr(cell), int(ei.size // 2), ei.ctypes.data_as(_pi32),
-                                          sh.ctypes.data_as(_pi32))
+                                          sh.ctypes.data_as(_pi32), None if kp is None else _dptr(kp))
             _check(self._h, rc, "rg_sk_velocity")
             return None
-        kp = None if kpoint is None else np.ascontiguousarray(kpoint, dtype=np.float64).reshape(3)
         nnzb = C.c_int64()
         rc = self._lib.rg_sk_assemble(self._h, int(slot), C.byref(t), int(types.size), types.ctypes.data_as(_pi32),
                                       _dptr(pos), _dptr(cell), int(ei.size // 2), ei.ctypes.data_as(_pi32),

@@ -1046,7 +1046,6 @@ int sk_build(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, c
         if (edge_index[e] < 0 || edge_index[e] >= n_atoms)
             return fail(ctx, RG_ERR_INVALID, "edge atom index out of range");
     const bool cplx = kpoint && (kpoint[0] != 0.0 || kpoint[1] != 0.0 || kpoint[2] != 0.0);
-    if (velocity && cplx) return fail(ctx, RG_ERR_UNSUPPORTED, "sk velocity: Gamma point only");
     RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     RG_HIP(ctx, sk_upload_constants(), "SK constants");
 
@@ -1185,8 +1184,18 @@ int sk_build(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, c
                     w = pos[3 * j + a] - pos[3 * i + a];
                     for (int r = 0; r < 3; ++r) w += edge_shift[3 * (int64_t)e + r] * cell[3 * r + a];
                 }
-                bond[a].push_back(w);
-                bond[a].push_back(0.0);
+                // at k != 0 the edge's Bloch phase multiplies its bond
+                // component, as it multiplies the block of H(k): B_a(k) is
+                // anti-Hermitian and V_a = i B_a Hermitian in H(k)'s gauge
+                double c = 1.0, sn = 0.0;
+                if (cplx && e >= 0) {
+                    double arg = 0.0;
+                    for (int k = 0; k < 3; ++k) arg += kpoint[k] * edge_shift[3 * (int64_t)e + k];
+                    c = std::cos(2.0 * M_PI * arg);
+                    sn = std::sin(2.0 * M_PI * arg);
+                }
+                bond[a].push_back(w * c);
+                bond[a].push_back(w * sn);
             }
         }
         if (cplx) {
@@ -1227,7 +1236,7 @@ int sk_build(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, c
                                   static_cast<const int32_t *>(dsrc.p), static_cast<const double2 *>(dph.p),
                                   static_cast<const int32_t *>(drow.p), static_cast<const int32_t *>(dtype.p),
                                   sd.onsite, static_cast<const double *>(deblk.p),
-                                  static_cast<double *>(s.vals64.p), velocity ? 2 : (cplx ? 1 : 0), ctx->stream);
+                                  static_cast<double *>(s.vals64.p), cplx ? 1 : (velocity ? 2 : 0), ctx->stream);
         if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
         if (he != hipSuccess) return hipfail(ctx, he, "sk assemble: gather");
         s.nbr = n_atoms;
@@ -1255,8 +1264,8 @@ int rg_sk_assemble(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_at
 
 int rg_sk_velocity(rg_context *ctx, const rg_sk_table *t, int64_t n_atoms, const int32_t *atom_type,
                    const double *pos, const double *cell, int64_t n_edges, const int32_t *edge_index,
-                   const int32_t *edge_shift) {
-    return sk_build(ctx, RG_SLOT_VX, t, n_atoms, atom_type, pos, cell, n_edges, edge_index, edge_shift, nullptr,
+                   const int32_t *edge_shift, const double *kpoint) {
+    return sk_build(ctx, RG_SLOT_VX, t, n_atoms, atom_type, pos, cell, n_edges, edge_index, edge_shift, kpoint,
                     true, nullptr);
 }
 
@@ -1292,7 +1301,8 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
         const Slot &V = ctx->slots[RG_SLOT_VX + a];
         if (!V.present) return fail(ctx, RG_ERR_STATE, "velocity slots are empty (rg_sk_velocity)");
         if (V.nbr != H.nbr || V.b != H.b) return fail(ctx, RG_ERR_INVALID, "velocity and H differ in shape");
-        if (V.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "velocity factors must be real (Gamma)");
+        if (V.is_complex != H.is_complex)
+            return fail(ctx, RG_ERR_INVALID, "velocity factors and H must both be real (Gamma) or both complex (k)");
     }
     if (tangent)
         for (int q = RG_SLOT_DH; q <= RG_SLOT_DVZ; ++q) {

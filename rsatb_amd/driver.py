@@ -314,7 +314,8 @@ class RSATB:
         xz, yy, yz, zz) (Greenwood fills the diagonal; Bastin all six, as the
         T = 0 conductivity against the Fermi energy), and the energies in eV. The left and right sequences are kept whole on
         the device (partition_sections and max_workspace_mb are accepted and
-        do not apply)."""
+        do not apply). At a k-point the velocity carries the same Bloch phase
+        as H(k)."""
         forms = {"greenwood": 0, "bastin": 1}
         if formulation not in forms:
             raise RuntimeError(f"Unknown dc formulation: {formulation}")
@@ -341,11 +342,11 @@ class RSATB:
         sigma = np.zeros((len(self._kstates), n_moments, 6))
         energy = None
         for k_idx, ks in enumerate(self._kstates):
-            if np.any(ks["k"]):
-                raise RuntimeError("compute_dc_spectrum: the velocity is built at Gamma only")
             self._bind_kpoint(k_idx)
+            # the velocity in the gauge of the bound H(k) (Bloch phase per edge)
             self._ctx.sk_velocity(self.model.arrays("H"), st.atom_type, st.positions, st.cell,
-                                  np.stack([e.src, e.dst], axis=1), e.shift)
+                                  np.stack([e.src, e.dst], axis=1), e.shift,
+                                  kpoint=ks["k"] if np.any(ks["k"]) else None)
             sigma[k_idx], energy = self._ctx.dc_spectrum(n_moments, forms[formulation], kernels[kernel], prefactor,
                                                          precision=prec)
         # e^2 / (hbar Angstrom) in S/m (rsatb/driver.py:393)

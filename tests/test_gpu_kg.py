@@ -306,3 +306,73 @@ def test_fp32_transport_against_fp64(formulation, kind, monkeypatch):
         assert np.max(np.abs(s32 - s64)) > 0.0  # the FP32 path ran
     scale = np.max(np.abs(spectra[None]))
     assert np.max(np.abs(spectra["4096"] - spectra[None])) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("formulation", [0, 1])
+def test_dc_spectrum_at_k_against_atomic_gauge_oracle(formulation):
+    """At k != 0 the device builds H(k) and B_a(k) with the lattice Bloch
+    phase e^{2 pi i k . shift} per edge. The oracle's atomic-gauge pair
+    (bloch_h_and_v, commutator_velocity.hpp:24-31, pinned by Hellmann-Feynman)
+    at k' = -2 pi cell^-1 k is the same pair conjugated by U = diag(e^{i
+    k_cart . r}); with the samples taken to U^H r the oracle's spectrum must
+    equal the device's (1e-10 relative), for Greenwood and Bastin."""
+    t = sp3_silicon_table()
+    pos, cell = silicon_cell()
+    ei, sh = periodic_edges(pos, cell, 3.2)
+    types = np.zeros(len(pos), dtype=np.int32)
+    a = t.arrays("H")
+    kfrac = np.array([0.13, -0.21, 0.07])
+    vecs = so.edge_vectors(pos, cell, ei, sh)
+    blocks = [so.pair_block(t, "H", 0, 0, v) for v in vecs]
+    blocks = [np.zeros((9, 9)) if bk is None else bk for bk in blocks]
+    onsite = [np.diag(np.asarray(a["onsite"])[:9]) for _ in range(len(pos))]
+    k_cart = 2 * np.pi * np.linalg.solve(cell, kfrac)
+    H, V = kg.bloch_h_and_v(len(pos), 9, onsite, ei[0], ei[1], vecs, blocks, -k_cart)
+    e = np.linalg.eigvalsh(H)
+    c0, half = 0.5 * (e[0] + e[-1]), 0.55 * (e[-1] - e[0])
+    M, R = 40, 3
+    c = _lib.Context(0)
+    try:
+        c.sk_assemble(_lib.SLOT_H, a, types, pos, cell, ei.T.copy(), sh, kpoint=kfrac)
+        c.sk_velocity(a, types, pos, cell, ei.T.copy(), sh, kpoint=kfrac)
+        assert c.bsr_info(_lib.SLOT_VX)["is_complex"]
+        c.set_operator(_lib.FORM_ORTHOGONAL, c0, half)
+        c.set_samples(H.shape[0], R, _lib.SAMPLES_PHASE, 9)
+        print("case: Si KG at k", kfrac, "formulation", formulation, flush=True)
+        sigma, energy = c.dc_spectrum(M, formulation, prefactor=1.0)
+        psi = c.get_samples()
+    finally:
+        c.close()
+    ph = np.repeat(np.exp(-1j * (pos @ k_cart)), 9)  # U^H on every orbital row
+    L, Rs = kg.sequences(H, V, ph[:, None] * psi, c0, half, M)
+    if formulation == 0:
+        ref = kg.greenwood_fftcheb(L, Rs, M, n_samples=R)
+        got = sigma[:, [0, 3, 5]]
+    else:
+        ref = kg.bastin_fftcheb(L, Rs, M, n_samples=R)
+        got = sigma
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) <= 1e-10 * scale
+
+
+def test_driver_dc_spectrum_kpoints():
+    """RSATB.compute_dc_spectrum over two k-points: the Gamma entry equals a
+    Gamma-only run, and the k != 0 entry is the device spectrum of H(k) with
+    its own-gauge velocity (finite, and different from Gamma's)."""
+    from rsatb_amd.driver import RSATB, SetupOptions
+    from rsatb_amd.lattice import Structure
+    from rsatb_amd.sktb import SktbModel
+    from sk_cases import silicon_params
+
+    basis, h, _ = silicon_params()
+    model = SktbModel(basis, h)
+    pos, cell = silicon_cell()
+    opts = dict(nsamples=2, compute_bounds=False, energy_bounds=(-20.0, 25.0), sample_kind="phase")
+    both = RSATB(Structure(pos, cell), model, SetupOptions(kpoints=[[0, 0, 0], [0.13, -0.21, 0.07]], **opts))
+    gamma = RSATB(Structure(pos, cell), model, SetupOptions(**opts))
+    s2, e2 = both.compute_dc_spectrum(32)
+    s1, e1 = gamma.compute_dc_spectrum(32)
+    assert s2.shape == (2, 32, 6)
+    np.testing.assert_allclose(s2[0], s1[0], rtol=0, atol=1e-12 * np.max(np.abs(s1)))
+    assert np.all(np.isfinite(s2[1]))
+    assert np.max(np.abs(s2[1] - s2[0])) > 1e-6 * np.max(np.abs(s2[0]))
