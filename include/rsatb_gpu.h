@@ -306,8 +306,8 @@ RG_API int rg_dc_spectrum_ex(rg_context *ctx, int M, int formulation, int kernel
  * per parameter gives the Jacobian d sigma / d theta, the derivative behind
  * gradients of conductivity losses. No counterpart in the reference (its
  * transport path has no derivatives); the spectrum it differentiates is
- * dc_spectrum_fftcheb (fftcheb_transport.hpp:543-653). Memory: 12 M x rows x R
- * vector elements. */
+ * dc_spectrum_fftcheb (fftcheb_transport.hpp:543-653). Gamma only (real H and
+ * tangent matrices). Memory: 12 M x rows x R vector elements. */
 RG_API int rg_dc_spectrum_tangent(rg_context *ctx, int M, int formulation, int kernel, double prefactor,
                                   double *sigma_out, double *dsigma_out, double *energy_out);
 

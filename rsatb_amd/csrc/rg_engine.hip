@@ -1304,6 +1304,8 @@ int dc_spectrum(rg_context *ctx, int M, int formulation, int kernel, double pref
         if (V.is_complex != H.is_complex)
             return fail(ctx, RG_ERR_INVALID, "velocity factors and H must both be real (Gamma) or both complex (k)");
     }
+    if (tangent && H.is_complex)
+        return fail(ctx, RG_ERR_UNSUPPORTED, "the tangent spectrum is built at Gamma (real H and tangent matrices)");
     if (tangent)
         for (int q = RG_SLOT_DH; q <= RG_SLOT_DVZ; ++q) {
             const Slot &D = ctx->slots[q];
