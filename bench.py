@@ -1,19 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the stochastic Chebyshev/KPM moment loop on one MI355X.
 
-Workload (one "step" = one full moment computation, kpm_dos_moments of
-rsatb/calc/dos.hpp:124-189, over all sample columns):
-  diamond, 8 x 8 x 8 conventional cells (4096 atoms), 13 orbitals per atom,
-  random symmetric H and S blocks to the 2.7 A second-neighbour cutoff, the
-  sandwich operator Z^H H Z with Z = graph-local S^-1/2 built on the GPU,
-  spectral window from GPU Lanczos + 0.5 eV padding, R = 64 random-phase
-  complex vectors, M = 2048 moments, FP64.
-The value is orbital x vector x moment units per second with all operands
-resident in HBM. Per-kernel times come from HIP events recorded on the
-library's stream over the timed steps; the CPU baseline is the oracle
-(the FP64 C restatement of the reference's loop) on a bounded sample.
+Headline workload (the largest single-GPU configuration):
+  diamond, 24 x 24 x 24 conventional cells (110,592 atoms, 1,437,696
+  orbitals), 13 orbitals per atom, random symmetric H and S blocks to the
+  2.7 A second-neighbour cutoff, the sandwich operator Z^H H Z with Z the
+  graph-local S^-1/2 built on the GPU, spectral window from GPU Lanczos +
+  0.5 eV padding, R = 128 random-phase complex vectors, M = 4096 moments.
+One timed step is a fixed slice of that recurrence at full N and R: the
+first 128 moments (64 doubled sandwich applies, kpm_dos_moments of
+rsatb/calc/dos.hpp:124-189 over all sample columns), FP64. The complete
+M = 4096 runs in FP64 and in the FP32 form (FP32 MFMA products and
+recurrence, FP64 dots) run once outside the timed steps and give the
+per-moment and DOS errors of the FP32 form. The value is orbital x vector x
+moment units per second with all operands resident in HBM. Per-kernel times
+come from HIP events recorded on the library's stream in a second pass over
+the same steps; the CPU baseline is the oracle (the FP64 C restatement of
+the reference's loop, in the reference's unfused form over complex storage
+and fused) on a bounded sample of the same workload. Secondary keys: the
+diamond 8^3 configuration (R = 64, M = 2048), the graphene 64 x 64 plain DOS
+(R = 16, M = 1024), the differentiable graphene 128 x 128 form with a
+learnable overlap, SK assembly, Kubo-Greenwood, the b = 1 graphene scale-up
+and the twisted-bilayer transport leg.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
@@ -38,14 +48,18 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    p.add_argument("--cells", type=int, default=8, help="diamond conventional cells per edge")
-    p.add_argument("--nsamples", type=int, default=64)
-    p.add_argument("--moments", type=int, default=2048)
-    p.add_argument("--cpu-moments", type=int, default=256,
-                   help="moments in the timed CPU sample (about 15 s on 16 host cores)")
+    p.add_argument("--cells", type=int, default=24, help="diamond conventional cells per edge (headline)")
+    p.add_argument("--nsamples", type=int, default=128)
+    p.add_argument("--step-moments", type=int, default=128, help="moments per timed step (a slice of the run)")
+    p.add_argument("--moments", type=int, default=4096, help="moments of the complete FP64 / FP32 runs")
+    p.add_argument("--no-full", action="store_true", help="skip the complete FP64 / FP32 runs")
+    p.add_argument("--cpu-moments", type=int, default=4,
+                   help="moments in each timed CPU sample (2 sandwich applies at the headline size)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    p.add_argument("--no-fp32", action="store_true", help="skip the FP32 steps")
+    p.add_argument("--no-legs", action="store_true", help="only the headline")
+    p.add_argument("--no-c2", action="store_true", help="skip the diamond 8^3 leg")
+    p.add_argument("--no-c1", action="store_true", help="skip the graphene 64x64 plain-DOS leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward leg")
     p.add_argument("--adj-cells", type=int, default=128, help="graphene rectangular cells per edge (adjoint leg)")
     p.add_argument("--no-sk", action="store_true", help="skip the Slater-Koster assembly leg")
@@ -63,6 +77,10 @@ def parse():
     p.add_argument("--tblg-samples", type=int, default=16)
     p.add_argument("--tblg-moments", type=int, default=512)
     return p.parse_args()
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
 
 
 PROFILES = Path(__file__).resolve().parent / "profiles"
@@ -110,8 +128,11 @@ def roofline(stats: dict, precision: str, workload_tag: str = "") -> dict:
 
 
 def cpu_baseline(calc, n_moments: int, ref_mu) -> dict:
-    """The oracle's fused FP64 loop on the host cores over the same operator,
-    the same samples and the first n_moments moments."""
+    """The oracle's FP64 loop on the host cores over the same operator and
+    the same samples, first n_moments moments, in the two modes the
+    reference's CPU path is measured in: the reference's own form (value;
+    unfused passes over complex<double> storage, rsatb/calc/dos.hpp:169-183,
+    cheby.hpp:589-590) and the fused loop."""
     import numpy as np
 
     from oracle import oracle as O
@@ -129,16 +150,22 @@ def cpu_baseline(calc, n_moments: int, ref_mu) -> dict:
     else:
         op = O.Operator(O.FORM_ORTHOGONAL, ks["H"], c, a)
     psi = ctx.get_samples()
-    op.moments(psi, 4, doubling=True, fused=True)  # warm-up
-    t0 = time.perf_counter()
-    mu = op.moments(psi, n_moments, doubling=True, fused=True)
-    dt = time.perf_counter() - t0
     rows, R = psi.shape
-    err = float(np.max(np.abs(mu - ref_mu[:n_moments])))
-    return {"value": rows * R * n_moments / dt, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": f"same operator and samples, first {n_moments} moments ({n_moments // 2} sandwich applies, "
-                      f"doubled), fused FP64 oracle loop, {dt:.2f} s",
-            "seconds": round(dt, 3), "max_abs_moment_diff_vs_gpu": err}
+    out = {}
+    for name, fused in (("reference_form", False), ("fused", True)):
+        t0 = time.perf_counter()
+        mu = op.moments(psi, n_moments, doubling=True, fused=fused)
+        dt = time.perf_counter() - t0
+        out[name] = {"value": rows * R * n_moments / dt, "seconds": round(dt, 3),
+                     "max_abs_moment_diff_vs_gpu": float(np.max(np.abs(mu - ref_mu[:n_moments])))}
+        log(f"cpu baseline {name}: {dt:.2f} s")
+    applies = 1 + max(0, (n_moments - 2) // 2)
+    return {"value": out["reference_form"]["value"], "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"same operator and samples, first {n_moments} moments ({applies} sandwich applies, doubled), "
+                      f"FP64 oracle on {threads} host threads, no warm-up; value = the reference's unfused form over "
+                      f"complex<double> storage ({out['reference_form']['seconds']} s), fused loop beside it "
+                      f"({out['fused']['seconds']} s)",
+            "reference_form": out["reference_form"], "fused": out["fused"]}
 
 
 def dos_error(mu_ref, mu):
@@ -155,28 +182,28 @@ def dos_error(mu_ref, mu):
 
 def adjoint_leg(cells: int, steps: int) -> dict:
     """Backward/forward time of the differentiable moments on graphene
-    (cells x cells rectangular, onsite + 3 hopping shells, NN overlap 0.1 with
-    Z held fixed), R = 32, M = 512, checkpoints every 32 steps."""
+    (cells x cells rectangular, onsite + 3 hopping shells and the
+    nearest-neighbour overlap s1 = 0.1, all learnable), R = 32, M = 512,
+    checkpoints every 32 steps. A forward builds S(s1), its graph-local factor
+    Z on the device and the sandwich moments; a backward runs the adjoint
+    recurrence, dL/dH, dL/dZ and dL/dS through the patches."""
     import numpy as np
 
     from rsatb_amd import _lib
-    from rsatb_amd.autograd import DifferentiableKPM, sk_pz_parameterization
+    from rsatb_amd.autograd import DifferentiableKPM, sk_pz_overlap_parameterization, sk_pz_parameterization
     from rsatb_amd.lattice import graphene_rect_supercell, neighbor_edges
-    from rsatb_amd.models import SkPzModel, assemble
 
     st = graphene_rect_supercell(cells, cells)
-    e = neighbor_edges(st, 2.9)
-    model = sk_pz_parameterization(st, e, [1.6, 2.5, 2.9])
-    theta = np.array([0.0, -2.7, -0.2, -0.18])
-    S = assemble(st, neighbor_edges(st, 1.6), SkPzModel(overlap_shells=[(1.6, 0.1, 0.0)]), "S")
+    model = sk_pz_parameterization(st, neighbor_edges(st, 2.9), [1.6, 2.5, 2.9])
+    overlap = sk_pz_overlap_parameterization(st, neighbor_edges(st, 1.6), [1.6])
+    theta = np.array([0.0, -2.7, -0.2, -0.18, 0.1])
     with _lib.Context(0) as c:
-        c.upload_bsr(_lib.SLOT_S, S)
+        c.upload_bsr(_lib.SLOT_S, overlap.build(theta[4:]))
         c.isqrt_patches()
-        Z = c.download_bsr(_lib.SLOT_Z)
-        c.upload_bsr(_lib.SLOT_H, model.build(theta))
+        c.upload_bsr(_lib.SLOT_H, model.build(theta[:4]))
         lo, hi = c.estimate_bounds(_lib.FORM_SANDWICH, steps=200, padding=0.5)
     M = 512
-    eng = DifferentiableKPM(model, 32, (lo, hi), M, checkpoint_stride=32, sample_kind="phase", Z=Z)
+    eng = DifferentiableKPM(model, 32, (lo, hi), M, checkpoint_stride=32, sample_kind="phase", overlap=overlap)
     g = np.linspace(1.0, -1.0, M)
     eng.forward(theta)
     eng.backward(g)
@@ -191,11 +218,125 @@ def adjoint_leg(cells: int, steps: int) -> dict:
         grad = eng.backward(g)
         t_b += time.perf_counter() - t1
     t_b /= steps
+    # the same pair with the factor held fixed (no overlap gradient)
+    Zfix = None
+    with _lib.Context(0) as c:
+        c.upload_bsr(_lib.SLOT_S, overlap.build(theta[4:]))
+        c.isqrt_patches()
+        Zfix = c.download_bsr(_lib.SLOT_Z)
     eng.ctx.close()
-    return {"workload": f"graphene {cells}x{cells} rectangular ({st.n_atoms} atoms), onsite + 3 hopping shells, "
-                        "NN overlap 0.1 (Z fixed), R=32 phase, M=512, checkpoints every 32 steps, FP64",
+    eng2 = DifferentiableKPM(model, 32, (lo, hi), M, checkpoint_stride=32, sample_kind="phase", Z=Zfix)
+    eng2.forward(theta[:4])
+    eng2.backward(g)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng2.forward(theta[:4])
+    t_f2 = (time.perf_counter() - t0) / steps
+    t_b2 = 0.0
+    for _ in range(steps):
+        eng2.forward(theta[:4])
+        t1 = time.perf_counter()
+        eng2.backward(g)
+        t_b2 += time.perf_counter() - t1
+    t_b2 /= steps
+    eng2.ctx.close()
+    return {"workload": f"graphene {cells}x{cells} rectangular ({st.n_atoms} atoms), onsite + 3 hopping shells + "
+                        "NN overlap s1=0.1 (all learnable; S^-1/2 patches rebuilt in every forward), R=32 phase, "
+                        "M=512, checkpoints every 32 steps, FP64",
             "forward_ms": round(t_f * 1e3, 3), "backward_ms": round(t_b * 1e3, 3),
-            "bwd_fwd": round(t_b / t_f, 3), "grad_theta": [float(x) for x in grad]}
+            "bwd_fwd": round(t_b / t_f, 3), "grad_theta": [float(x) for x in grad],
+            "params": list(model.names) + list(overlap.names),
+            "fixed_Z": {"forward_ms": round(t_f2 * 1e3, 3), "backward_ms": round(t_b2 * 1e3, 3),
+                        "bwd_fwd": round(t_b2 / t_f2, 3)}}
+
+
+def c1_leg(reps: int = 5) -> dict:
+    """The plain-DOS graphene configuration at its own size: 64 x 64
+    rectangular cells (16,384 atoms, p_z), t = -2.7 eV nearest neighbours,
+    onsite 0 stored, window b = 0, a = 8.1 / 0.99, R = 16 Rademacher samples
+    (seed 20240917), M = 1024 doubled moments, Jackson DOS on 2001 points;
+    FP64 and FP32. Wall time per run against the summed kernel time of the
+    same run (the launch gaps of a small operator)."""
+    import numpy as np
+
+    from rsatb_amd import _lib
+    from rsatb_amd.lattice import graphene_rect_supercell, neighbor_edges
+    from rsatb_amd.models import SkPzModel, assemble
+
+    st = graphene_rect_supercell(64, 64)
+    H = assemble(st, neighbor_edges(st, 1.6), SkPzModel())
+    N, R, M = st.n_atoms, 16, 1024
+    x = -1.0 + np.arange(2001) / 1000.0
+    out = {"workload": f"graphene 64x64 rectangular cells ({N} atoms, p_z), t=-2.7 eV, window b=0 a=8.1/0.99, "
+                       f"R={R} Rademacher (seed 20240917), M={M} doubled moments, Jackson DOS on 2001 points"}
+    with _lib.Context(0) as c:
+        c.upload_bsr(_lib.SLOT_H, H)
+        c.set_operator(_lib.FORM_ORTHOGONAL, 0.0, 8.1 / 0.99)
+        c.set_samples(N, R, _lib.SAMPLES_RADEMACHER, 20240917)
+        mus = {}
+        for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
+            c.kpm_moments(M, prec)  # warm-up
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                mus[name] = c.kpm_moments(M, prec)
+            dt = (time.perf_counter() - t0) / reps
+            c.set_profiling(True)
+            c.kpm_moments(M, prec)
+            ks = {nm: c.kernel_stats(k) for k, nm in _lib.KCLASS_NAMES.items()}
+            c.set_profiling(False)
+            kern = sum(v["avg_ms"] * v["count"] for v in ks.values()) * 1e-3
+            out[name] = {"ms": round(dt * 1e3, 3), "value": float(N) * R * M / dt, "unit": UNIT,
+                         "kernel_ms": round(kern * 1e3, 3), "wall_over_kernel": round(dt / kern, 3),
+                         "launches": int(sum(v["count"] for v in ks.values()))}
+    rho = _lib.kpm_density(mus["fp64"], x, 0.5 / 1000.0)
+    out["dos_err_fp32"] = dos_error(mus["fp64"], mus["fp32"])
+    out["dos_max"] = float(np.max(2.0 * rho))
+    return out
+
+
+def c2_leg(reps: int) -> dict:
+    """Diamond 8 x 8 x 8 conventional cells (53,248 orbitals), sandwich,
+    R = 64 random-phase complex vectors, M = 2048: complete runs in FP64 and
+    in the FP32 form, with the roofline of the FP64 run's dominant kernel."""
+    import numpy as np
+
+    from rsatb_amd import _lib
+    from rsatb_amd.driver import RSATB, PreconditionerOptions, SetupOptions
+    from rsatb_amd.lattice import diamond_supercell
+    from rsatb_amd.models import RandomBlockModel
+
+    st = diamond_supercell(8)
+    calc = RSATB(st, RandomBlockModel(), SetupOptions(
+        nsamples=64, sample_seed=20240917, compute_bounds=True, sample_kind="phase",
+        preconditioner=PreconditionerOptions(operator_form="sandwich")))
+    calc._bind_kpoint(0)
+    ctx = calc.context
+    rows, R, M = st.n_atoms * 13, 64, 2048
+    out = {"workload": f"diamond 8x8x8 conventional cells ({st.n_atoms} atoms x 13 orbitals), sandwich Z^H H Z, "
+                       f"R={R} random-phase complex vectors, M={M} moments"}
+    mus = {}
+    for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
+        ctx.kpm_moments(M, prec)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mus[name] = ctx.kpm_moments(M, prec)
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"ms": round(dt * 1e3, 2), "value": rows * R * M / dt}
+        if name == "fp64":
+            ctx.set_profiling(True)
+            ctx.kpm_moments(M, prec)
+            stats = {}
+            for k, nm in _lib.KCLASS_NAMES.items():
+                st_ = ctx.kernel_stats(k)
+                st_["name"] = nm
+                stats[nm] = st_
+            ctx.set_profiling(False)
+            out["roofline"] = roofline(stats, "fp64", "diamond8_R64_sandwich")
+    out["fp32_speedup"] = round(out["fp32"]["value"] / out["fp64"]["value"], 3)
+    out["fp32_max_abs_dmu"] = float(np.max(np.abs(mus["fp32"] - mus["fp64"])))
+    out["fp32_dos_err"] = dos_error(mus["fp64"], mus["fp32"])
+    ctx.close()
+    return out
 
 
 def sk_silicon_params():
@@ -486,6 +627,18 @@ def tblg_leg(reps: int, R: int, M: int) -> dict:
             "sigma_matches_plain_call": bool(np.max(np.abs(sigma - sigma0)) <= 1e-10 * np.max(np.abs(sigma0)))}
 
 
+def per_moment_error(mu_ref, mu, blocks: int = 16) -> dict:
+    """|mu_n - mu_ref_n| over all moments: its maximum, the moment where it
+    occurs, and the maximum within each of `blocks` equal ranges of n."""
+    import numpy as np
+
+    d = np.abs(mu - mu_ref)
+    edges = np.linspace(0, d.size, blocks + 1).astype(int)
+    return {"max": float(np.max(d)), "argmax": int(np.argmax(d)),
+            "block_max": [float(np.max(d[edges[k]:edges[k + 1]])) for k in range(blocks)],
+            "block_edges": [int(e) for e in edges]}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -503,75 +656,108 @@ def main() -> int:
     st = diamond_supercell(args.cells)
     calc = RSATB(st, RandomBlockModel(), SetupOptions(
         nsamples=args.nsamples, sample_seed=20240917, compute_bounds=True, sample_kind="phase",
-        preconditioner=PreconditionerOptions(operator_form="sandwich"), precision=args.precision))
+        preconditioner=PreconditionerOptions(operator_form="sandwich")))
     calc._bind_kpoint(0)
     ctx = calc.context
     setup_s = time.perf_counter() - t_setup
-    prec = _lib.PREC_FP32 if args.precision == "fp32" else _lib.PREC_FP64
     rows = st.n_atoms * 13
-    M = args.moments
-    print(f"setup {setup_s:.1f}s: {rows} orbitals, bounds {calc.energy_bounds()}, patches {calc._kstates[0].get('isqrt')}",
-          file=sys.stderr, flush=True)
+    R, Ms = args.nsamples, args.step_moments
+    log(f"setup {setup_s:.1f}s: {rows} orbitals, bounds {calc.energy_bounds()}, "
+        f"patches {calc._kstates[0].get('isqrt')}")
+    prec64 = _lib.PREC_FP64
 
     for _ in range(args.warmup):
-        mu = ctx.kpm_moments(M, prec)
+        ctx.kpm_moments(Ms, prec64)
+    log(f"warm-up done ({args.warmup} steps)")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        mu = ctx.kpm_moments(M, prec)  # synchronous on return
+        mu_step = ctx.kpm_moments(Ms, prec64)  # synchronous on return
     dt = time.perf_counter() - t0
-    # The same K runs again with a HIP event pair around every launch on the
+    log(f"timed FP64 steps: {dt / args.steps * 1e3:.1f} ms per step")
+    # The same K steps again with a HIP event pair around every launch on the
     # library's stream: the per-kernel durations behind the roofline. This
     # pass is timed separately, because the events idle the GPU a few us at
-    # every recorded launch (about 8 us here, 12 % of the FP64 run).
+    # every recorded launch.
     ctx.set_profiling(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.kpm_moments(M, prec)
+        ctx.kpm_moments(Ms, prec64)
     dt_prof = time.perf_counter() - t0
     stats = {}
     for k, name in _lib.KCLASS_NAMES.items():
-        s = ctx.kernel_stats(k)
-        s["name"] = name
-        stats[name] = s
+        s_ = ctx.kernel_stats(k)
+        s_["name"] = name
+        stats[name] = s_
     ctx.set_profiling(False)
     ms = dt / args.steps * 1e3
-    value = rows * args.nsamples * M / (dt / args.steps)
+    value = rows * R * Ms / (dt / args.steps)
+    tag = f"diamond{args.cells}_R{R}_sandwich"
 
     line = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms, "ms_per_step_event_timed": dt_prof / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 (f64 dots)", "data": "synthetic",
+        "ms_per_step": ms, "ms_per_step_event_timed": dt_prof / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"diamond {args.cells}x{args.cells}x{args.cells} conventional cells "
-                               f"({st.n_atoms} atoms x 13 orbitals), sandwich Z^H H Z, R={args.nsamples} "
-                               f"random-phase complex vectors, M={M} moments",
-                   "n_orbitals": rows, "R": args.nsamples, "M": M, "form": "sandwich",
-                   "precision": args.precision, "bounds": list(calc.energy_bounds())},
-        "roofline": roofline(stats, args.precision, f"diamond{args.cells}_R{args.nsamples}_sandwich"),
+                               f"({st.n_atoms} atoms x 13 orbitals = {rows} orbitals), sandwich Z^H H Z with "
+                               f"graph-local Z = S^-1/2, R={R} random-phase complex vectors, M={args.moments} "
+                               f"moments; one step = the first {Ms} moments ({Ms // 2} doubled sandwich applies) "
+                               f"at full N and R",
+                   "n_orbitals": rows, "R": R, "M": args.moments, "step_moments": Ms, "form": "sandwich",
+                   "precision": "fp64", "bounds": list(calc.energy_bounds()), "setup_s": round(setup_s, 1)},
+        "roofline": roofline(stats, "fp64", tag),
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "count": v["count"]} for k, v in stats.items()},
-        "mu0": float(mu[0].real), "max_abs_mu": float(np.max(np.abs(mu))),
+        "mu0": float(mu_step[0].real),
     }
     if not args.no_fp32:
-        ctx.kpm_moments(M, _lib.PREC_FP32)
+        ctx.kpm_moments(Ms, _lib.PREC_FP32)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            mu32 = ctx.kpm_moments(M, _lib.PREC_FP32)
+            ctx.kpm_moments(Ms, _lib.PREC_FP32)
         dt32 = time.perf_counter() - t0
-        v32 = rows * args.nsamples * M / (dt32 / args.steps)
+        v32 = rows * R * Ms / (dt32 / args.steps)
         line["fp32"] = {"value": v32, "ms_per_step": dt32 / args.steps * 1e3, "speedup_vs_fp64": v32 / value,
-                        "max_abs_dmu": float(np.max(np.abs(mu32 - mu))), "dos_err": dos_error(mu, mu32),
-                        "dos_err_definition": "max_E |rho32 - rho64| / max_E rho64, Jackson, 2001 points"}
-    if not args.no_adjoint:
-        line["adjoint"] = adjoint_leg(args.adj_cells, max(1, args.steps))
-    if not args.no_sk:
-        line["sk_assembly"] = sk_leg(args.sk_cells, 3, args.sk_cpu_edges)
-    if not args.no_kg:
-        line["kubo_greenwood"] = kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)
-    if not args.no_graphene_scale:
-        line["graphene_scale"] = graphene_scale_leg(args.graphene_cells)
-    if not args.no_tblg:
-        line["tblg_transport"] = tblg_leg(args.tblg_reps, args.tblg_samples, args.tblg_moments)
+                        "dtype": "f32 MFMA products and recurrence, f64 dots"}
+        log(f"timed FP32 steps: {dt32 / args.steps * 1e3:.1f} ms per step")
+    mu64 = None
+    if not args.no_full:
+        M = args.moments
+        t0 = time.perf_counter()
+        mu64 = ctx.kpm_moments(M, prec64)
+        t64 = time.perf_counter() - t0
+        full = {"fp64_s": round(t64, 3), "fp64_value": rows * R * M / t64}
+        log(f"complete FP64 run (M={M}): {t64:.2f} s")
+        if not args.no_fp32:
+            t0 = time.perf_counter()
+            mu32 = ctx.kpm_moments(M, _lib.PREC_FP32)
+            t32 = time.perf_counter() - t0
+            log(f"complete FP32 run (M={M}): {t32:.2f} s")
+            full.update({"fp32_s": round(t32, 3), "fp32_value": rows * R * M / t32,
+                         "fp32_speedup": round(t64 / t32, 3), "dos_err": dos_error(mu64, mu32),
+                         "dos_err_definition": "max_E |rho32 - rho64| / max_E rho64, Jackson, 2001 points",
+                         "per_moment_abs_err": per_moment_error(mu64, mu32)})
+        line["full_run"] = full
+        line["mu0_full"] = float(mu64[0].real)
+        line["max_abs_mu"] = float(np.max(np.abs(mu64)))
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(calc, args.cpu_moments, mu)
+        ref = mu64 if mu64 is not None else mu_step
+        line["cpu_baseline"] = cpu_baseline(calc, min(args.cpu_moments, len(ref)), ref)
+    ctx.close()
+    del calc
+    if not args.no_legs:
+        legs = [("c2", not args.no_c2, lambda: c2_leg(3)),
+                ("c1", not args.no_c1, lambda: c1_leg()),
+                ("adjoint", not args.no_adjoint, lambda: adjoint_leg(args.adj_cells, max(1, min(args.steps, 5)))),
+                ("sk_assembly", not args.no_sk, lambda: sk_leg(args.sk_cells, 3, args.sk_cpu_edges)),
+                ("kubo_greenwood", not args.no_kg,
+                 lambda: kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)),
+                ("graphene_scale", not args.no_graphene_scale, lambda: graphene_scale_leg(args.graphene_cells)),
+                ("tblg_transport", not args.no_tblg,
+                 lambda: tblg_leg(args.tblg_reps, args.tblg_samples, args.tblg_moments))]
+        for key, on, fn in legs:
+            if on:
+                t0 = time.perf_counter()
+                line[key] = fn()
+                log(f"leg {key}: {time.perf_counter() - t0:.1f} s")
     print(json.dumps(line), flush=True)
     return 0
 

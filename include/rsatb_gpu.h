@@ -176,11 +176,36 @@ RG_API int rg_set_checkpointing(rg_context *ctx, int stride);
  * column-mean moments: dL_dmu[m] = dL/dRe(mu_m), m < M. Writes dL/dH for every
  * stored entry of the real H in RG_SLOT_H (nnzb*b*b doubles, row-major within
  * blocks), the window (center, half_width) held fixed; for the sandwich form
- * Z is held fixed. The adjoint states obey
+ * Z is held fixed (rg_kpm_moments_backward_ex also differentiates Z and S).
+ * RG_ERR_STATE when the operator, the window or the samples changed after the
+ * forward pass. The adjoint states obey
  *   lam_n = 2 A~ lam_{n+1} - lam_{n+2} + dL/dpsi_n (direct),
  * dL/dA~ = lam_1 psi_0^T + 2 sum_{n>=2} lam_n psi_{n-1}^T, summed over columns;
  * segments between checkpoints are recomputed forward on the device. */
 RG_API int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_out);
+
+/* The same reverse pass with the gradient per BSR slot (the per-slot
+ * dL_dvalues of the differentiable form); each output is optional (null
+ * skips it), at least one is required:
+ *   grad_h: dL/dH on RG_SLOT_H's pattern, as rg_kpm_moments_backward;
+ *   grad_z: dL/dZ on RG_SLOT_Z's pattern (sandwich form, real Z), with the
+ *           two factors tied (Z^H = Z^T): dL/dZ = (H Z G + H Z G^T) / a,
+ *           G = dL/dA~ as above;
+ *   grad_s: dL/dS on RG_SLOT_S's pattern through the graph-local factor
+ *           (graph_inverse_sqrt's seed, rsatb/calc/isqrtm.hpp:149-232, as
+ *           rg_isqrt_patches builds it): needs Z and Z^H from rg_isqrt_patches
+ *           of the overlap now in RG_SLOT_S. Per patch A_i = S[C_i, C_i],
+ *           dL/dA_i = L_f(A_i, E_i) with f(x) = x^{-1/2} (the Frechet
+ *           derivative, self-adjoint for symmetric A_i) and E_i = dL/dZ's
+ *           block row i in the rows of block i, from the tangent of the same
+ *           batched Newton-Schulz iteration on the device; every patch entry is
+ *           added to the stored block it came from.
+ * All outputs nnzb*b*b doubles, row-major within blocks; per-entry values
+ * whose symmetric part (G_ij + G_ji) is what a symmetric parameterization
+ * sees. RG_ERR_STATE when H, Z, Z^H, S, the window or the samples changed
+ * after the forward pass. */
+RG_API int rg_kpm_moments_backward_ex(rg_context *ctx, const double *dL_dmu, double *grad_h, double *grad_z,
+                                      double *grad_s);
 
 /* ---- DOS task (ComputeDos kpm branch, observables.hpp:311-444) ---- */
 

@@ -392,3 +392,82 @@ def dense_moment_gradient(A: np.ndarray, psi, g: np.ndarray) -> np.ndarray:
         if gm != 0.0 and m > 0:
             F += gm * chebyshev_divided_differences(w, m)
     return V @ (F * Wt) @ V.T
+
+
+# ------------------------------------------ gradient through the patches
+def _patch_of(S, i):
+    """(block columns C_i, dense A_i = S[C_i, C_i], position of block i)."""
+    b = S.block_dim
+    s0, s1 = int(S.row_ptr[i]), int(S.row_ptr[i + 1])
+    ci = S.col_ind[s0:s1].astype(np.int64)
+    pos = {int(c): k for k, c in enumerate(ci)}
+    A = np.zeros((ci.size * b, ci.size * b))
+    for k, c in enumerate(ci):
+        for s in range(int(S.row_ptr[c]), int(S.row_ptr[c + 1])):
+            cc = int(S.col_ind[s])
+            if cc in pos:
+                kk = pos[cc]
+                A[k * b:(k + 1) * b, kk * b:(kk + 1) * b] = S.values[s]
+    return ci, pos, A
+
+
+def isqrt_patches_vjp(S, dZ: np.ndarray) -> np.ndarray:
+    """dL/dS on S's pattern ([nnzb, b, b]) for dL/dZ on Z's (= S's) pattern,
+    through isqrt_patches (real S). Per patch, A = S[C_i, C_i] = V diag(w) V^T
+    and X = A^{-1/2}: dL/dA = V (F o (V^T E V)) V^T (Daleckii-Krein), with E =
+    dL/dX (dL/dZ's block row i in the rows of block i) and F the divided
+    differences of f(x) = x^{-1/2}, in the cancellation-free form
+    (f(a) - f(b)) / (a - b) = -1 / (sqrt(a) sqrt(b) (sqrt(a) + sqrt(b)))
+    (= f'(a) at a = b). Each entry of dL/dA is added to the stored block of S
+    it was gathered from."""
+    b = S.block_dim
+    out = np.zeros_like(S.values, dtype=np.float64)
+    for i in range(S.n_block_rows):
+        s0 = int(S.row_ptr[i])
+        ci, pos, A = _patch_of(S, i)
+        n = A.shape[0]
+        E = np.zeros((n, n))
+        me = pos[i]
+        for k in range(ci.size):
+            E[me * b:(me + 1) * b, k * b:(k + 1) * b] = dZ[s0 + k]
+        w, V = np.linalg.eigh(0.5 * (A + A.T))
+        sq = np.sqrt(w)
+        F = -1.0 / (sq[:, None] * sq[None, :] * (sq[:, None] + sq[None, :]))
+        dA = V @ (F * (V.T @ E @ V)) @ V.T
+        for k, c in enumerate(ci):
+            for s in range(int(S.row_ptr[c]), int(S.row_ptr[c + 1])):
+                cc = int(S.col_ind[s])
+                if cc in pos:
+                    kk = pos[cc]
+                    out[s] += dA[k * b:(k + 1) * b, kk * b:(kk + 1) * b]
+    return out
+
+
+def _on_pattern(P, D: np.ndarray) -> np.ndarray:
+    """Entries of the dense matrix D at the stored blocks of P ([nnzb, b, b])."""
+    b = P.block_dim
+    out = np.zeros((int(P.row_ptr[-1]), b, b))
+    for i in range(P.n_block_rows):
+        for s in range(int(P.row_ptr[i]), int(P.row_ptr[i + 1])):
+            j = int(P.col_ind[s])
+            out[s] = D[i * b:(i + 1) * b, j * b:(j + 1) * b]
+    return out
+
+
+def sandwich_moment_gradients(H, S, center: float, half_width: float, psi, g: np.ndarray) -> dict:
+    """Gradients of L = sum_m g_m Re mu_m for the sandwich operator
+    A~ = (Z^T H Z - center) / half_width with Z = isqrt_patches(S), from the
+    dense eigendecomposition: G = dL/dA~ (dense_moment_gradient, symmetric),
+    dL/dH = Z G Z^T / a, dL/dZ = (H Z G + H Z G^T) / a, dL/dS =
+    isqrt_patches_vjp(S, dL/dZ); each on its matrix's pattern, plus the
+    moments themselves. Real H and S."""
+    Z = isqrt_patches(S)
+    Zd, Hd = Z.to_dense(), H.to_dense()
+    n = Hd.shape[0]
+    A = (Zd.T @ Hd @ Zd - center * np.eye(n)) / half_width
+    G = dense_moment_gradient(A, psi, g)
+    dH = Zd @ G @ Zd.T / half_width
+    dZ = (Hd @ Zd @ G + Hd @ Zd @ G.T) / half_width
+    dZp = _on_pattern(Z, dZ)
+    return {"H": _on_pattern(H, dH), "Z": dZp, "S": isqrt_patches_vjp(S, dZp),
+            "mu": dense_moments(A, psi, len(g)), "Z_bsr": Z}

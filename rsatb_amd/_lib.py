@@ -83,6 +83,7 @@ SIGNATURES = {
     "rg_estimate_bounds": (_i, [_vp, _i, _i, _u64, _d, _pd, _pd]),
     "rg_set_checkpointing": (_i, [_vp, _i]),
     "rg_kpm_moments_backward": (_i, [_vp, _pd, _pd]),
+    "rg_kpm_moments_backward_ex": (_i, [_vp, _pd, _pd, _pd, _pd]),
     "rg_cancel": (_i, [_vp]),
     "rg_set_profiling": (_i, [_vp, _i]),
     "rg_kernel_stats": (_i, [_vp, _i, _pd, _pi64, _pd, _pd]),
@@ -419,6 +420,24 @@ class Context:
         rc = self._lib.rg_kpm_moments_backward(self._h, _dptr(g), _dptr(out))
         _check(self._h, rc, "rg_kpm_moments_backward")
         return out
+
+    def kpm_moments_backward_ex(self, dL_dmu: np.ndarray, want_h: bool = True, want_z: bool = False,
+                                want_s: bool = False) -> dict:
+        """Per-slot gradients {"H", "Z", "S"} ([nnzb, b, b] each, upload layout)
+        of the requested slots (rg_kpm_moments_backward_ex)."""
+        g = np.ascontiguousarray(np.asarray(dL_dmu, dtype=np.float64))
+        outs = {}
+        ptrs = []
+        for name, slot, want in (("H", SLOT_H, want_h), ("Z", SLOT_Z, want_z), ("S", SLOT_S, want_s)):
+            if want:
+                info = self.bsr_info(slot)
+                outs[name] = np.zeros((info["nnzb"], info["block_dim"], info["block_dim"]))
+                ptrs.append(_dptr(outs[name]))
+            else:
+                ptrs.append(None)
+        rc = self._lib.rg_kpm_moments_backward_ex(self._h, _dptr(g), *ptrs)
+        _check(self._h, rc, "rg_kpm_moments_backward_ex")
+        return outs
 
     def compute_dos(self, emin: float, emax: float, de: float, precision: int = PREC_FP64):
         npts, _ = dos_plan(emin, emax, de)

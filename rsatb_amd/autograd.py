@@ -1,14 +1,17 @@
 """Differentiable KPM moments: a torch.autograd.Function over the library's
-adjoint Chebyshev recurrence (rg_kpm_moments_backward); and a differentiable
-DC conductivity spectrum over its tangent recurrence (rg_dc_spectrum_tangent).
+adjoint Chebyshev recurrence (rg_kpm_moments_backward_ex); and a
+differentiable DC conductivity spectrum over its tangent recurrence
+(rg_dc_spectrum_tangent).
 
 PyTorch is used only here, as the autograd wrapper: the forward and the
 backward both run in the gfx950 library; torch sees the moments (a real
-vector) and hands back dL/dmu. Parameters enter through a linear
-parameterization of the Hamiltonian on a fixed sparsity pattern,
-H(theta) = sum_k theta_k B_k, so dL/dtheta = B^T dL/dH. The spectral window
-is held fixed at the value given (it must contain the spectrum for every
-theta the caller evaluates), and for an overlap the factor Z is held fixed.
+vector) and hands back dL/dmu. Parameters enter through affine
+parameterizations on fixed sparsity patterns, H(theta) = sum_k theta_k B_k
+and, for a non-orthogonal basis, S(theta) = S0 + sum_k theta_k C_k, so
+dL/dtheta = B^T dL/dH + C^T dL/dS; dL/dS runs through the graph-local factor
+Z = S^{-1/2} the library builds on the device (rg_isqrt_patches). The spectral
+window is held fixed at the value given (it must contain the spectrum for
+every theta the caller evaluates).
 """
 from __future__ import annotations
 
@@ -24,15 +27,20 @@ from .lattice import Edges, Structure
 
 @dataclass
 class LinearOperatorModel:
-    """H(theta) = sum_k theta_k B_k; basis: [nnzb * b * b, n_params] dense,
-    rows in the pattern's entry order (row-major within blocks)."""
+    """A(theta) = offset + sum_k theta_k B_k; basis: [nnzb * b * b, n_params]
+    dense, rows in the pattern's entry order (row-major within blocks);
+    offset (optional, same layout) the constant part, e.g. an overlap's
+    identity."""
 
     pattern: BSR
     basis: np.ndarray
     names: tuple
+    offset: np.ndarray | None = None
 
     def build(self, theta) -> BSR:
         vals = self.basis @ np.asarray(theta, dtype=np.float64)
+        if self.offset is not None:
+            vals = vals + self.offset
         b = self.pattern.block_dim
         return BSR(self.pattern.n_block_rows, b, self.pattern.row_ptr, self.pattern.col_ind,
                    vals.reshape(-1, b, b))
@@ -72,6 +80,36 @@ def sk_pz_parameterization(st: Structure, edges: Edges, shell_radii) -> LinearOp
         cols_basis.append(Hp.values.reshape(-1))
     names = ("e0",) + tuple(f"t{k + 1}" for k in range(len(radii)))
     return LinearOperatorModel(pattern, np.stack(cols_basis, axis=1), names)
+
+
+def sk_pz_overlap_parameterization(st: Structure, edges: Edges, shell_radii) -> LinearOperatorModel:
+    """p_z overlap S = I + sum_k s_k (1 - n^2) on the bonds of shell k
+    (d <= shell_radii[k], first match; n = z / d), the Slater-Koster pp_pi
+    overlap with unit onsite overlap: the affine model of the learnable
+    nearest-neighbour overlap s1."""
+    radii = list(shell_radii)
+    n = st.n_atoms
+    shell = np.full(edges.src.size, -1)
+    for k, r in sorted(enumerate(radii), key=lambda t: t[1], reverse=True):
+        shell[edges.dist <= r] = k
+    keep = shell >= 0
+    src, dst, sh = edges.src[keep], edges.dst[keep], shell[keep]
+    n2 = (edges.vec[keep, 2] / edges.dist[keep]) ** 2
+    rows = np.concatenate([np.arange(n), src])
+    cols = np.concatenate([np.arange(n), dst])
+    coef0 = np.zeros(rows.size)
+    coef0[:n] = 1.0
+    S0 = BSR.from_blocks(n, 1, rows, cols, coef0.reshape(-1, 1, 1))
+    cols_basis = []
+    for k in range(len(radii)):
+        coef = np.zeros(rows.size)
+        coef[n:] = np.where(sh == k, 1.0 - n2, 0.0)
+        Sk = BSR.from_blocks(n, 1, rows, cols, coef.reshape(-1, 1, 1))
+        if not (np.array_equal(Sk.row_ptr, S0.row_ptr) and np.array_equal(Sk.col_ind, S0.col_ind)):
+            raise RuntimeError("parameter patterns differ")
+        cols_basis.append(Sk.values.reshape(-1))
+    names = tuple(f"s{k + 1}" for k in range(len(radii)))
+    return LinearOperatorModel(S0, np.stack(cols_basis, axis=1), names, offset=S0.values.reshape(-1).copy())
 
 
 @dataclass
@@ -185,17 +223,31 @@ class DifferentiableConductivity:
 
 
 class DifferentiableKPM:
-    """Moments mu(theta) on one device context with a backward pass."""
+    """Moments mu(theta) on one device context with a backward pass.
+
+    theta = (H parameters of `model`, then the overlap parameters of
+    `overlap` when given). With an overlap model every forward builds S(theta)
+    and its graph-local factor Z on the device (rg_isqrt_patches) and runs the
+    sandwich form; the backward returns dL/dtheta through H and through S -> Z
+    (rg_kpm_moments_backward_ex). A fixed factor Z (`Z`) gives the sandwich
+    form with no overlap gradient. Every forward is stamped: a backward for an
+    earlier forward (two nodes of one torch graph) re-runs that forward first,
+    so each node's gradient uses its own states."""
 
     def __init__(self, model: LinearOperatorModel, nsamples: int, window, n_moments: int,
                  checkpoint_stride: int = 32, sample_kind: str = "rademacher", seed: int = 20240917,
-                 Z: BSR | None = None, device: int = 0):
+                 Z: BSR | None = None, overlap: LinearOperatorModel | None = None, isqrt_tol: float = 1e-12,
+                 isqrt_maxit: int = 60, device: int = 0):
+        if Z is not None and overlap is not None:
+            raise RuntimeError("give a fixed factor Z or an overlap model, not both")
         self.model = model
+        self.overlap = overlap
         self.M = int(n_moments)
         emin, emax = window
         if not (math.isfinite(emin) and math.isfinite(emax) and emin < emax):
             raise RuntimeError("window must be finite with emin < emax")
         self.window = (float(emin), float(emax))
+        self.isqrt = (float(isqrt_tol), int(isqrt_maxit))
         self.ctx = _lib.Context(device)
         rows = model.pattern.n_rows
         kind = _lib.SAMPLES_PHASE if sample_kind == "phase" else _lib.SAMPLES_RADEMACHER
@@ -205,25 +257,53 @@ class DifferentiableKPM:
             self.ctx.upload_bsr(_lib.SLOT_Z, Z)
             self.ctx.upload_bsr(_lib.SLOT_ZH, Z.conj_transpose())
             form = _lib.FORM_SANDWICH
+        if overlap is not None:
+            if overlap.pattern.n_rows != rows:
+                raise RuntimeError("the overlap model and the Hamiltonian model differ in rows")
+            form = _lib.FORM_SANDWICH
         self.form = form
         self.ctx.set_checkpointing(checkpoint_stride)
         self._theta = None
+        self._stamp = 0
+
+    @property
+    def n_params(self) -> int:
+        return self.model.basis.shape[1] + (0 if self.overlap is None else self.overlap.basis.shape[1])
+
+    def _split(self, theta):
+        nh = self.model.basis.shape[1]
+        return theta[:nh], theta[nh:]
 
     def forward(self, theta) -> np.ndarray:
         theta = np.asarray(theta, dtype=np.float64)
-        self.ctx.upload_bsr(_lib.SLOT_H, self.model.build(theta))
+        if theta.size != self.n_params:
+            raise RuntimeError(f"theta has {theta.size} entries; the models take {self.n_params}")
+        th_h, th_s = self._split(theta)
+        if self.overlap is not None:
+            self.ctx.upload_bsr(_lib.SLOT_S, self.overlap.build(th_s))
+            self.ctx.isqrt_patches(*self.isqrt)
+        self.ctx.upload_bsr(_lib.SLOT_H, self.model.build(th_h))
         emin, emax = self.window
         self.ctx.set_operator(self.form, (emax + emin) / 2.0, (emax - emin) / 2.0)
         mu = self.ctx.kpm_moments(self.M, _lib.PREC_FP64, _lib.DOUBLING_ON)
         self._theta = theta.copy()
+        self._stamp += 1
         return mu.real.copy()
 
-    def backward(self, dL_dmu) -> np.ndarray:
-        """dL/dtheta for the last forward()."""
+    def backward(self, dL_dmu, stamp: int | None = None, theta=None) -> np.ndarray:
+        """dL/dtheta for the last forward(), or for the forward `stamp` (at
+        `theta`), re-run first when a later forward replaced its states."""
         if self._theta is None:
             raise RuntimeError("backward() needs a forward() first")
-        g = self.ctx.kpm_moments_backward(np.asarray(dL_dmu, dtype=np.float64))
-        return self.model.contract(g)
+        if stamp is not None and stamp != self._stamp:
+            if theta is None:
+                raise RuntimeError("a backward for an earlier forward needs that forward's theta")
+            self.forward(theta)
+        g = np.asarray(dL_dmu, dtype=np.float64)
+        if self.overlap is None:
+            return self.model.contract(self.ctx.kpm_moments_backward(g))
+        outs = self.ctx.kpm_moments_backward_ex(g, want_h=True, want_s=True)
+        return np.concatenate([self.model.contract(outs["H"]), self.overlap.contract(outs["S"])])
 
     def moments(self, theta):
         """torch entry point: mu(theta) with autograd."""
@@ -242,13 +322,16 @@ def _kpm_fn():
     class KPMMoments(torch.autograd.Function):
         @staticmethod
         def forward(ctx, theta, engine):
-            mu = engine.forward(theta.detach().cpu().double().numpy())
+            th = theta.detach().cpu().double().numpy()
+            mu = engine.forward(th)
             ctx.engine = engine
+            ctx.stamp = engine._stamp
+            ctx.theta = th.copy()
             return torch.from_numpy(mu)
 
         @staticmethod
         def backward(ctx, grad_mu):
-            g = ctx.engine.backward(grad_mu.detach().cpu().double().numpy())
+            g = ctx.engine.backward(grad_mu.detach().cpu().double().numpy(), stamp=ctx.stamp, theta=ctx.theta)
             return torch.from_numpy(g), None
 
     _FN = KPMMoments

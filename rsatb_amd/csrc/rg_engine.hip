@@ -27,9 +27,29 @@ using namespace rg;
 namespace {
 
 
+// Owning device allocation: freed when the owner goes away (a context member
+// at rg_destroy, a local at scope exit, error returns included). Move-only.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     hipError_t ensure(size_t n) {
         if (n <= bytes && p) return hipSuccess;
         if (p) {
@@ -61,6 +81,9 @@ struct Slot {
     // gather table of the MFMA SpMM: xrow[s * b + kk] = col_ind[s] * b + kk
     DevBuf xrow;
     bool has_xrow = false;
+    // changes on every write of the slot (context-wide counter), so a pass
+    // that keeps state derived from the slot can tell it is stale
+    uint64_t gen = 0;
 };
 
 struct KStat {
@@ -108,6 +131,10 @@ struct rg_context {
 
     std::atomic<int> cancel{0};
 
+    // generation counter behind Slot::gen, window_gen and samples_gen
+    uint64_t gen_counter = 0;
+    uint64_t window_gen = 0, samples_gen = 0;
+
     // forward states kept for rg_kpm_moments_backward
     int ckpt_stride = 0;
     DevBuf ckpt;
@@ -115,6 +142,23 @@ struct rg_context {
     bool ckpt_valid = false;
     int ckpt_M = 0, ckpt_N = 0, ckpt_R = 0;
     bool ckpt_complex = false;
+    // what the kept states were computed from: form, rows, generations of
+    // H, Z, Z^H, the window and the samples
+    int ckpt_form = 0;
+    int64_t ckpt_rows = 0;
+    uint64_t ckpt_gen[5] = {0, 0, 0, 0, 0};
+    // backward work buffers (kept between calls): three adjoint states, a
+    // zero vector, the recomputed segment, its Z and H Z products, and the
+    // gradients on the patterns of H and Z
+    DevBuf bw_lam[3], bw_zero, bw_seg, bw_zseg, bw_hzseg, bw_grad, bw_gradz;
+    // graph-local factor provenance: Z, Z^H came from rg_isqrt_patches of S
+    // at generation z_src_gen and were written at generation z_gen
+    uint64_t z_src_gen = 0, z_gen = 0, zh_gen = 0;
+    int64_t patch_ld = 0;
+    double patch_tol = 0.0;
+    int patch_maxit = 0;
+    // patch adjoint work buffers (kept between calls)
+    DevBuf pt_buf[10], pt_cs, pt_it, pt_res, pt_dS;
 
     bool prof = false;
     KStat stats[RG_KCLASS_COUNT];
@@ -128,6 +172,10 @@ int fail(rg_context *ctx, int code, const std::string &msg) {
     if (ctx) ctx->err = msg;
     return code;
 }
+
+// Marks a slot as rewritten (its derived state, such as kept forward states,
+// goes stale).
+void touch_slot(rg_context *ctx, Slot &s) { s.gen = ++ctx->gen_counter; }
 
 int hipfail(rg_context *ctx, hipError_t e, const char *what) {
     return fail(ctx, e == hipErrorOutOfMemory ? RG_ERR_NOMEM : RG_ERR_HIP,
@@ -311,19 +359,22 @@ int group_dots(rg_context *ctx, const GroupDots &gd, VecMode vm, int R, int ndot
 // cheby.hpp:555-591): Orthogonal H x; Sandwich Z^H (H (Z x)).
 // zx (sandwich only, optional): where Z x is written instead of the scratch
 // vector, for callers that reuse it.
+// hzx (sandwich only, optional): where H Z x is written instead of the
+// second scratch vector.
 int apply_chain(rg_context *ctx, VecMode vm, const void *x, void *y, int R, EpiArgs e,
-                int *grid_out, int form = -1, void *zx = nullptr) {
+                int *grid_out, int form = -1, void *zx = nullptr, void *hzx = nullptr) {
     if (form < 0) form = ctx->form;
     e.xs = x;
     if (form == RG_FORM_ORTHOGONAL) return spmm(ctx, RG_SLOT_H, vm, x, y, R, e, grid_out);
     EpiArgs plain;
     plain.mode = EPI_PLAIN;
     void *t1 = zx ? zx : ctx->vec[3].p;
+    void *t2 = hzx ? hzx : ctx->vec[4].p;
     int rc = spmm(ctx, RG_SLOT_Z, vm, x, t1, R, plain, nullptr);
     if (rc) return rc;
-    rc = spmm(ctx, RG_SLOT_H, vm, t1, ctx->vec[4].p, R, plain, nullptr);
+    rc = spmm(ctx, RG_SLOT_H, vm, t1, t2, R, plain, nullptr);
     if (rc) return rc;
-    return spmm(ctx, RG_SLOT_ZH, vm, ctx->vec[4].p, y, R, e, grid_out);
+    return spmm(ctx, RG_SLOT_ZH, vm, t2, y, R, e, grid_out);
 }
 
 int check_operator(rg_context *ctx) {
@@ -413,6 +464,17 @@ int dev_vec_to_host(rg_context *ctx, const void *src, int64_t rows, int R, bool 
 
 int64_t op_rows(const rg_context *ctx) {
     return ctx->slots[RG_SLOT_H].nbr * (int64_t)ctx->slots[RG_SLOT_H].b;
+}
+
+// Generations of everything the moment loop reads: H, Z, Z^H (sandwich
+// only), the window and the samples.
+void forward_gens(const rg_context *ctx, uint64_t g[5]) {
+    const bool sw = ctx->form == RG_FORM_SANDWICH;
+    g[0] = ctx->slots[RG_SLOT_H].gen;
+    g[1] = sw ? ctx->slots[RG_SLOT_Z].gen : 0;
+    g[2] = sw ? ctx->slots[RG_SLOT_ZH].gen : 0;
+    g[3] = ctx->window_gen;
+    g[4] = ctx->samples_gen;
 }
 
 // Moments from the dots each step stored (dos.hpp:150-187, pdos.hpp:240-258):
@@ -559,6 +621,9 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
             ctx->ckpt_N = Npsi;
             ctx->ckpt_R = R;
             ctx->ckpt_complex = vm.complex_vec;
+            ctx->ckpt_form = ctx->form;
+            ctx->ckpt_rows = rows;
+            forward_gens(ctx, ctx->ckpt_gen);
         }
     } else {
         e.mode = EPI_PLAINSTEP;
@@ -658,27 +723,9 @@ void rg_destroy(rg_context *ctx) {
         (void)hipEventDestroy(pe.stop);
     }
     for (auto ev : ctx->pool) (void)hipEventDestroy(ev);
-    for (auto &s : ctx->slots) {
-        s.row_ptr.release();
-        s.col_ind.release();
-        s.vals64.release();
-        s.vals32.release();
-        s.xrow.release();
-    }
-    ctx->psi.release();
-    ctx->ckpt.release();
-    for (auto &v : ctx->vec) v.release();
-    ctx->partials.release();
-    ctx->raw.release();
-    for (DevBuf *b : {&ctx->pd_rows, &ctx->pd_chunk, &ctx->pd_grp, &ctx->pd_partial, &ctx->pd_out})
-        b->release();
-    for (DevBuf *b : {&ctx->sk_ints, &ctx->sk_dbls, &ctx->sk_type, &ctx->sk_pos, &ctx->sk_ei, &ctx->sk_sh,
-                      &ctx->sk_eblk, &ctx->sk_flags, &ctx->sk_sp, &ctx->sk_src, &ctx->sk_ph, &ctx->sk_row,
-                      &ctx->kg_store, &ctx->kg_mu})
-        b->release();
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    delete ctx; // every device buffer is owned by a DevBuf member and freed here
 }
 
 const char *rg_last_error(const rg_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -702,6 +749,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     Slot &s = ctx->slots[slot];
     s.present = false;
     s.vals32_valid = false;
+    touch_slot(ctx, s);
     const size_t nv = (size_t)nnzb * b * b * (dtype == RG_C128 ? 2 : 1);
     RG_HIP(ctx, s.row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc row_ptr");
     RG_HIP(ctx, s.col_ind.ensure((size_t)nnzb * 4 + 4), "hipMalloc col_ind");
@@ -741,6 +789,7 @@ int rg_set_operator(rg_context *ctx, int form, double center, double half_width)
     ctx->center = center;
     ctx->half_width = half_width;
     ctx->op_set = true;
+    ctx->window_gen = ++ctx->gen_counter;
     return RG_OK;
 }
 
@@ -757,6 +806,7 @@ int rg_set_samples(rg_context *ctx, int64_t n_rows, int R, int kind, uint64_t se
     const bool cplx = (kind == RG_SAMPLES_PHASE || kind == RG_SAMPLES_UPLOAD_COMPLEX);
     const int64_t n = n_rows * (int64_t)R;
     ctx->samples_set = false;
+    ctx->samples_gen = ++ctx->gen_counter;
     RG_HIP(ctx, ctx->psi.ensure((size_t)n * (cplx ? 16 : 8)), "hipMalloc samples");
     if (kind == RG_SAMPLES_RADEMACHER || kind == RG_SAMPLES_PHASE) {
         const double scale = 1.0 / std::sqrt((double)n_rows);
@@ -1223,6 +1273,7 @@ int sk_build(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, c
         Slot &s = ctx->slots[sl];
         s.present = false;
         s.vals32_valid = false;
+        touch_slot(ctx, s);
         he = up(dph, w.data(), w.size() * 8);
         if (he == hipSuccess) he = s.row_ptr.ensure((size_t)(n_atoms + 1) * 8);
         if (he == hipSuccess) he = s.col_ind.ensure((size_t)nnzb * 4 + 4);
@@ -1767,24 +1818,31 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         }
     }
     const int n = (int)(nbmax * b);
-    const int ld = ((n + 63) / 64) * 64;
+    const bool small = n <= ns_small_max();
+    const int ld = small ? ns_small_max() : ((n + 63) / 64) * 64;
     if (ld > 512)
         return fail(ctx, RG_ERR_UNSUPPORTED,
                     "patch dimension " + std::to_string(n) + " exceeds the 512 the kernel supports");
     const size_t mat = (size_t)ld * ld;
     const size_t budget = (size_t)6 << 30;
-    int64_t batch = (int64_t)std::max<size_t>(1, budget / (5 * mat * 8));
+    int64_t batch = small ? nbr : (int64_t)std::max<size_t>(1, budget / (5 * mat * 8));
     batch = std::min<int64_t>(batch, nbr);
-    DevBuf buf[5], cs, resid;
-    for (auto &bb : buf) RG_HIP(ctx, bb.ensure((size_t)batch * mat * 8), "hipMalloc patches");
+    DevBuf buf[5], cs, resid, iters;
+    if (!small)
+        for (auto &bb : buf) RG_HIP(ctx, bb.ensure((size_t)batch * mat * 8), "hipMalloc patches");
     RG_HIP(ctx, cs.ensure((size_t)batch * 8), "hipMalloc");
     RG_HIP(ctx, resid.ensure((size_t)batch * 8), "hipMalloc");
+    if (small) RG_HIP(ctx, iters.ensure((size_t)batch * 4), "hipMalloc");
     // the factor shares S's pattern
     Slot &Z = ctx->slots[RG_SLOT_Z];
     Slot &ZH = ctx->slots[RG_SLOT_ZH];
+    ctx->z_src_gen = 0;
+    ctx->z_gen = 0;
+    ctx->zh_gen = 0;
     for (Slot *t : {&Z, &ZH}) {
         t->present = false;
         t->vals32_valid = false;
+        touch_slot(ctx, *t);
         RG_HIP(ctx, t->row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc");
         RG_HIP(ctx, t->col_ind.ensure((size_t)S.nnzb * 4 + 4), "hipMalloc");
         RG_HIP(ctx, t->vals64.ensure((size_t)S.nnzb * b * b * 8), "hipMalloc");
@@ -1801,7 +1859,33 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
     int it_max = 0;
     double res_max = 0.0;
     std::vector<double> hres((size_t)batch);
-    for (int64_t first = 0; first < nbr; first += batch) {
+    if (small) {
+        // every patch in one launch, each iterating to its own convergence
+        RG_HIP(ctx, launch_ns_small(0, nbr, b, (const int64_t *)S.row_ptr.p, (const int32_t *)S.col_ind.p,
+                                    (const double *)S.vals64.p, tol, maxit, (double *)Z.vals64.p, (double *)cs.p,
+                                    (int *)iters.p, (double *)resid.p, ctx->stream),
+               "small patch iteration");
+        std::vector<int> hit((size_t)nbr);
+        RG_HIP(ctx, hipMemcpyAsync(hres.data(), resid.p, (size_t)nbr * 8, hipMemcpyDeviceToHost, ctx->stream),
+               "download");
+        RG_HIP(ctx, hipMemcpyAsync(hit.data(), iters.p, (size_t)nbr * 4, hipMemcpyDeviceToHost, ctx->stream),
+               "download");
+        RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        for (int64_t q = 0; q < nbr; ++q) {
+            const double r = hres[(size_t)q];
+            if (!(r <= 100.0))
+                return fail(ctx, RG_ERR_INVALID,
+                            "Newton-Schulz diverged on the overlap patches (||I - ZY||_F = " + std::to_string(r) +
+                                "); is S positive definite?");
+            if (!(r < tol))
+                return fail(ctx, RG_ERR_INVALID,
+                            "Newton-Schulz did not reach tol " + std::to_string(tol) + " in " + std::to_string(maxit) +
+                                " iterations (||I - ZY||_F = " + std::to_string(r) + ")");
+            it_max = std::max(it_max, hit[(size_t)q]);
+            res_max = std::max(res_max, r);
+        }
+    }
+    for (int64_t first = 0; first < (small ? 0 : nbr); first += batch) {
         const int64_t cnt = std::min<int64_t>(batch, nbr - first);
         double *Y = (double *)buf[0].p, *Zm = (double *)buf[1].p, *T = (double *)buf[2].p;
         double *Y2 = (double *)buf[3].p, *Z2 = (double *)buf[4].p;
@@ -1851,8 +1935,13 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
                                    (const double *)Z.vals64.p, (double *)ZH.vals64.p, ctx->stream),
            "conjugate transpose");
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
-    for (auto &bb : buf) bb.release();
     Z.present = ZH.present = true;
+    ctx->z_src_gen = S.gen;
+    ctx->z_gen = Z.gen;
+    ctx->zh_gen = ZH.gen;
+    ctx->patch_ld = ld;
+    ctx->patch_tol = tol;
+    ctx->patch_maxit = maxit;
     if (iters_out) *iters_out = it_max;
     if (resid_out) *resid_out = res_max;
     return RG_OK;
@@ -1990,19 +2079,156 @@ int rg_set_checkpointing(rg_context *ctx, int stride) {
     return RG_OK;
 }
 
-int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_out) {
-    if (!ctx || !dL_dmu || !grad_out) return RG_ERR_INVALID;
+namespace {
+
+// dS (column-major blocks on S's pattern, device) = dL/dS from dL/dZ (dz,
+// column-major blocks on Z's pattern = S's pattern) through the graph-local
+// map S -> Z of rg_isqrt_patches. Z's block row i is the rows of block i of
+// X_i = A_i^{-1/2}, A_i = S[C_i, C_i]; for symmetric A_i the Frechet
+// derivative L_f(A_i, .) of f(x) = x^{-1/2} is self-adjoint, so
+// dL/dA_i = L_f(A_i, E_i), E_i = dL/dX_i (dL/dZ's block row i placed in the
+// rows of block i). It is the tangent of the same coupled iteration:
+// with Y0 = A/c, Y0' = E, Z0' = 0 the tangent Z' converges to
+// L_f(A/c, E) = c^{3/2} L_f(A, E). dL/dS then sums every patch's entries onto
+// the stored blocks they came from.
+int patch_backward(rg_context *ctx, const double *dz, double **dS_out) {
+    const Slot &S = ctx->slots[RG_SLOT_S];
+    const int64_t nbr = S.nbr;
+    const int b = S.b;
+    const int ld = (int)ctx->patch_ld;
+    const bool small = ld == ns_small_max();
+    const size_t mat = (size_t)ld * ld;
+    const int extra = 2; // updates after the pair converged: the tangent settles with it
+    RG_HIP(ctx, ctx->pt_dS.ensure((size_t)S.nnzb * b * b * 8 + 8), "hipMalloc dL/dS");
+    double *dS = (double *)ctx->pt_dS.p;
+    RG_HIP(ctx, launch_zero(dS, S.nnzb * b * b, ctx->stream), "zero");
+    const int64_t *rp = (const int64_t *)S.row_ptr.p;
+    const int32_t *ci = (const int32_t *)S.col_ind.p;
+    const double *sv = (const double *)S.vals64.p;
+    const size_t budget = (size_t)6 << 30;
+    const int64_t batch =
+        small ? nbr : std::min<int64_t>(nbr, (int64_t)std::max<size_t>(1, budget / (10 * mat * 8)));
+    RG_HIP(ctx, ctx->pt_cs.ensure((size_t)batch * 8), "hipMalloc");
+    RG_HIP(ctx, ctx->pt_res.ensure((size_t)batch * 8), "hipMalloc");
+    RG_HIP(ctx, ctx->pt_it.ensure((size_t)batch * 4), "hipMalloc");
+    double *cs = (double *)ctx->pt_cs.p, *res = (double *)ctx->pt_res.p;
+    const double tol = ctx->patch_tol;
+    const int maxit = ctx->patch_maxit;
+    std::vector<double> hres((size_t)batch);
+    auto not_converged = [&](double worst) {
+        return fail(ctx, RG_ERR_INVALID,
+                    "the tangent Newton-Schulz iteration did not converge on the overlap patches (||I - ZY||_F = " +
+                        std::to_string(worst) + ")");
+    };
+    if (small) {
+        RG_HIP(ctx, ctx->pt_buf[0].ensure((size_t)nbr * mat * 8), "hipMalloc patch tangents");
+        double *dP = (double *)ctx->pt_buf[0].p;
+        RG_HIP(ctx, launch_ns_small_tangent(0, nbr, b, rp, ci, sv, dz, tol, maxit, extra, dP, cs, (int *)ctx->pt_it.p,
+                                            res, ctx->stream),
+               "patch tangent");
+        RG_HIP(ctx, launch_patch_grad_gather(nbr, b, S.nnzb, rp, ci, 0, nbr, ld, dP, cs, dS, ctx->stream),
+               "patch gradient gather");
+        RG_HIP(ctx, hipMemcpyAsync(hres.data(), res, (size_t)nbr * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+        RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        for (int64_t q = 0; q < nbr; ++q)
+            if (!(hres[(size_t)q] < tol)) return not_converged(hres[(size_t)q]);
+        *dS_out = dS;
+        return RG_OK;
+    }
+    for (auto &bb : ctx->pt_buf) RG_HIP(ctx, bb.ensure((size_t)batch * mat * 8), "hipMalloc patch tangents");
+    for (int64_t first = 0; first < nbr; first += batch) {
+        const int64_t cnt = std::min<int64_t>(batch, nbr - first);
+        double *Y = (double *)ctx->pt_buf[0].p, *Zm = (double *)ctx->pt_buf[1].p, *T = (double *)ctx->pt_buf[2].p;
+        double *Yd = (double *)ctx->pt_buf[3].p, *Zd = (double *)ctx->pt_buf[4].p, *Td = (double *)ctx->pt_buf[5].p;
+        double *Y2 = (double *)ctx->pt_buf[6].p, *Z2 = (double *)ctx->pt_buf[7].p;
+        double *Yd2 = (double *)ctx->pt_buf[8].p, *Zd2 = (double *)ctx->pt_buf[9].p;
+        RG_HIP(ctx, launch_patch_gather(first, cnt, b, rp, ci, sv, ld, Y, Zm, cs, ctx->stream), "patch gather");
+        RG_HIP(ctx, launch_patch_seed(first, cnt, b, rp, ci, dz, ld, Yd, Zd, ctx->stream), "patch seed");
+        bool conv = false;
+        int left = extra;
+        double worst = 0.0;
+        for (int it = 0;; ++it) {
+            RG_HIP(ctx, launch_zero(res, cnt, ctx->stream), "zero");
+            RG_HIP(ctx, launch_ns_gemm2(0, cnt, ld, Zm, Y, nullptr, nullptr, T, res, ctx->stream), "NS gemm");
+            RG_HIP(ctx, hipMemcpyAsync(hres.data(), res, (size_t)cnt * 8, hipMemcpyDeviceToHost, ctx->stream),
+                   "download");
+            RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+            worst = 0.0;
+            for (int64_t q = 0; q < cnt; ++q) {
+                if (!std::isfinite(hres[(size_t)q])) { worst = INFINITY; break; }
+                worst = std::max(worst, std::sqrt(hres[(size_t)q]));
+            }
+            if (!conv && worst < tol) conv = true;
+            if (conv) {
+                if (left == 0) break;
+                --left;
+            } else if (!(worst <= 100.0) || it >= maxit) {
+                return not_converged(worst);
+            }
+            RG_HIP(ctx, launch_ns_gemm2(2, cnt, ld, Zd, Y, Zm, Yd, Td, nullptr, ctx->stream), "NS gemm");
+            RG_HIP(ctx, launch_ns_gemm2(1, cnt, ld, Y, T, nullptr, nullptr, Y2, nullptr, ctx->stream), "NS gemm");
+            RG_HIP(ctx, launch_ns_gemm2(3, cnt, ld, Yd, T, Y, Td, Yd2, nullptr, ctx->stream), "NS gemm");
+            RG_HIP(ctx, launch_ns_gemm2(1, cnt, ld, T, Zm, nullptr, nullptr, Z2, nullptr, ctx->stream), "NS gemm");
+            RG_HIP(ctx, launch_ns_gemm2(3, cnt, ld, Td, Zm, T, Zd, Zd2, nullptr, ctx->stream), "NS gemm");
+            std::swap(Y, Y2);
+            std::swap(Zm, Z2);
+            std::swap(Yd, Yd2);
+            std::swap(Zd, Zd2);
+        }
+        RG_HIP(ctx, launch_patch_grad_gather(nbr, b, S.nnzb, rp, ci, first, cnt, ld, Zd, cs, dS, ctx->stream),
+               "patch gradient gather");
+    }
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    *dS_out = dS;
+    return RG_OK;
+}
+
+// device column-major blocks -> host row-major blocks
+int download_blocks(rg_context *ctx, const Slot &s, const void *dev, double *out) {
+    const int b = s.b;
+    const int64_t ng = s.nnzb * b * b;
+    std::vector<double> cm((size_t)ng);
+    RG_HIP(ctx, hipMemcpyAsync(cm.data(), dev, (size_t)ng * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    for (int64_t q = 0; q < s.nnzb; ++q)
+        for (int r = 0; r < b; ++r)
+            for (int cc = 0; cc < b; ++cc) out[(q * b + r) * b + cc] = cm[(q * b + cc) * b + r];
+    return RG_OK;
+}
+
+// The adjoint pass behind rg_kpm_moments_backward(_ex).
+int kpm_backward(rg_context *ctx, const double *dL_dmu, double *grad_h, double *grad_z, double *grad_s) {
     if (!ctx->ckpt_valid)
         return fail(ctx, RG_ERR_STATE,
                     "no forward states: call rg_set_checkpointing(stride > 0), then a doubled FP64 "
                     "rg_kpm_moments, before the backward pass");
     int rc = check_operator(ctx);
     if (rc) return rc;
+    {
+        uint64_t g[5];
+        forward_gens(ctx, g);
+        if (ctx->form != ctx->ckpt_form || op_rows(ctx) != ctx->ckpt_rows || ctx->R != ctx->ckpt_R ||
+            std::memcmp(g, ctx->ckpt_gen, sizeof(g)) != 0)
+            return fail(ctx, RG_ERR_STATE,
+                        "the kept forward states are stale: the operator form, H, Z, Z^H, the window or the "
+                        "samples changed after the forward pass; run rg_kpm_moments again");
+    }
     const Slot &Hs = ctx->slots[RG_SLOT_H];
     if (Hs.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "the backward pass needs a real Hamiltonian");
+    const bool sandwich = ctx->form == RG_FORM_SANDWICH;
+    const bool want_z = grad_z || grad_s;
+    if (want_z && !sandwich) return fail(ctx, RG_ERR_INVALID, "dL/dZ and dL/dS need the sandwich form");
+    const Slot &Zs = ctx->slots[RG_SLOT_Z];
+    if (want_z && Zs.is_complex) return fail(ctx, RG_ERR_UNSUPPORTED, "dL/dZ needs a real factor Z");
+    if (grad_s) {
+        const Slot &Ss = ctx->slots[RG_SLOT_S];
+        if (!Ss.present || ctx->z_src_gen == 0 || ctx->z_src_gen != Ss.gen || ctx->z_gen != Zs.gen ||
+            ctx->zh_gen != ctx->slots[RG_SLOT_ZH].gen)
+            return fail(ctx, RG_ERR_STATE,
+                        "dL/dS needs Z and Z^H built by rg_isqrt_patches from the overlap now in RG_SLOT_S");
+    }
     RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     const int M = ctx->ckpt_M, N = ctx->ckpt_N, R = ctx->ckpt_R, K = ctx->ckpt_stride;
-    const bool sandwich = ctx->form == RG_FORM_SANDWICH;
     VecMode vm;
     vm.complex_vec = ctx->ckpt_complex;
     const int64_t rows = op_rows(ctx);
@@ -2016,45 +2242,53 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
     for (int n = 2; n <= N; ++n)
         if (2 * n - 1 < M) G1 -= gd[2 * n - 1];
 
-    // buffers: 3 adjoint states, a zero vector, K + 2 recomputed forward
-    // states, two sandwich transforms, the gradient on H's pattern
-    DevBuf lam[3], zero, seg, grad;
-    for (auto &l : lam) RG_HIP(ctx, l.ensure(nb), "hipMalloc adjoint");
-    RG_HIP(ctx, zero.ensure(nb), "hipMalloc");
-    RG_HIP(ctx, seg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
+    // buffers (kept in the context): 3 adjoint states, a zero vector, K + 2
+    // recomputed forward states, their Z (and H Z) products, the gradients
+    for (auto &l : ctx->bw_lam) RG_HIP(ctx, l.ensure(nb), "hipMalloc adjoint");
+    RG_HIP(ctx, ctx->bw_zero.ensure(nb), "hipMalloc");
+    RG_HIP(ctx, ctx->bw_seg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
     if (sandwich) {
         RG_HIP(ctx, ctx->vec[3].ensure(nb), "hipMalloc");
         RG_HIP(ctx, ctx->vec[4].ensure(nb), "hipMalloc");
+        RG_HIP(ctx, ctx->bw_zseg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
     }
+    if (want_z) RG_HIP(ctx, ctx->bw_hzseg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
     const int64_t ng = Hs.nnzb * Hs.b * Hs.b;
-    RG_HIP(ctx, grad.ensure((size_t)ng * 8 + 8), "hipMalloc gradient");
-    RG_HIP(ctx, launch_zero((double *)zero.p, nvec, ctx->stream), "zero");
-    RG_HIP(ctx, launch_zero((double *)grad.p, ng, ctx->stream), "zero");
-    for (auto &l : lam) RG_HIP(ctx, launch_zero((double *)l.p, nvec, ctx->stream), "zero");
+    const int64_t ngz = want_z ? Zs.nnzb * Zs.b * Zs.b : 0;
+    RG_HIP(ctx, ctx->bw_grad.ensure((size_t)ng * 8 + 8), "hipMalloc gradient");
+    if (want_z) RG_HIP(ctx, ctx->bw_gradz.ensure((size_t)ngz * 8 + 8), "hipMalloc gradient");
+    void *zero = ctx->bw_zero.p, *seg = ctx->bw_seg.p, *zseg = ctx->bw_zseg.p, *hzseg = ctx->bw_hzseg.p;
+    RG_HIP(ctx, launch_zero((double *)zero, nvec, ctx->stream), "zero");
+    RG_HIP(ctx, launch_zero((double *)ctx->bw_grad.p, ng, ctx->stream), "zero");
+    if (want_z) RG_HIP(ctx, launch_zero((double *)ctx->bw_gradz.p, ngz, ctx->stream), "zero");
+    for (auto &l : ctx->bw_lam) RG_HIP(ctx, launch_zero((double *)l.p, nvec, ctx->stream), "zero");
 
     const double a = ctx->half_width, c = ctx->center;
     auto stored = [&](int j) -> void * {
         const int sl = ctx->ckpt_slot[j];
         return sl < 0 ? nullptr : static_cast<char *>(ctx->ckpt.p) + (size_t)sl * nb;
     };
-    // adjoint states: lam_{n+2}, lam_{n+1}, lam_n rotate through lam[]
-    void *l2 = lam[0].p, *l1 = lam[1].p, *l0 = lam[2].p;
-    // sandwich: Z psi_j of the segment, mostly produced by the recompute
-    // chains themselves (their first factor), the rest by one Z product each
-    DevBuf zseg;
-    if (sandwich) RG_HIP(ctx, zseg.ensure((size_t)(K + 2) * nb), "hipMalloc segment");
+    // adjoint states: lam_{n+2}, lam_{n+1}, lam_n rotate through bw_lam
+    void *l2 = ctx->bw_lam[0].p, *l1 = ctx->bw_lam[1].p, *l0 = ctx->bw_lam[2].p;
     EpiArgs plain;
     plain.mode = EPI_PLAIN;
-    auto accum = [&](const void *u, const void *v, double coef) -> int {
-        RG_HIP(ctx, launch_grad_accum(Hs.nbr, Hs.b, Hs.nnzb, (const int64_t *)Hs.row_ptr.p,
-                                      (const int32_t *)Hs.col_ind.p, (const double *)u,
-                                      (const double *)v, R, vm.complex_vec, coef,
-                                      (double *)grad.p, ctx->stream),
+    auto accum_on = [&](const Slot &P, void *G, const void *u, const void *v, double coef) -> int {
+        RG_HIP(ctx, launch_grad_accum(P.nbr, P.b, P.nnzb, (const int64_t *)P.row_ptr.p,
+                                      (const int32_t *)P.col_ind.p, (const double *)u, (const double *)v, R,
+                                      vm.complex_vec, coef, (double *)G, ctx->stream),
                "gradient accumulation");
         return RG_OK;
     };
+    auto accum = [&](const void *u, const void *v, double coef) { return accum_on(Hs, ctx->bw_grad.p, u, v, coef); };
+    // dL/dZ = (H Z G + H Z G^T) / a on Z's pattern for the pair (lam, psi)
+    // of dL/dA~ = G = sum coef lam psi^T (Z^H tied to Z^T)
+    auto accum_z = [&](const void *hzl, const void *psi, const void *hzp, const void *lam, double coef) -> int {
+        int r2 = accum_on(Zs, ctx->bw_gradz.p, hzl, psi, coef);
+        if (r2) return r2;
+        return accum_on(Zs, ctx->bw_gradz.p, hzp, lam, coef);
+    };
     const int nseg = N / K;
-    void *zpsi0 = nullptr;
+    void *zpsi0 = nullptr, *hzpsi0 = nullptr;
     for (int k = nseg; k >= 0; --k) {
         const int lo = std::max(1, k * K), hi = std::min(k * K + K - 1, N);
         if (lo > hi) continue;
@@ -2062,9 +2296,8 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
         const int jlo = lo - 1, jhi = std::min(hi + 1, N);
         std::vector<void *> ptr((size_t)(jhi - jlo + 1), nullptr);
         std::vector<bool> zdone((size_t)(jhi - jlo + 1), false);
-        auto zp_of = [&](int j) -> void * {
-            return static_cast<char *>(zseg.p) + (size_t)(j - jlo) * nb;
-        };
+        auto zp_of = [&](int j) -> void * { return static_cast<char *>(zseg) + (size_t)(j - jlo) * nb; };
+        auto hz_of = [&](int j) -> void * { return static_cast<char *>(hzseg) + (size_t)(j - jlo) * nb; };
         int next_seg = 0;
         EpiArgs ef;
         ef.mode = EPI_STEP; // its dots land in the partial buffer and are not read
@@ -2075,10 +2308,10 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
             void *p = stored(j);
             if (!p) {
                 if (j < 2 || j - 2 < jlo) return fail(ctx, RG_ERR_STATE, "checkpoint layout broken");
-                p = static_cast<char *>(seg.p) + (size_t)(next_seg++) * nb;
+                p = static_cast<char *>(seg) + (size_t)(next_seg++) * nb;
                 ef.xp = ptr[j - 2 - jlo];
                 rc = apply_chain(ctx, vm, ptr[j - 1 - jlo], p, R, ef, nullptr, -1,
-                                 sandwich ? zp_of(j - 1) : nullptr);
+                                 sandwich ? zp_of(j - 1) : nullptr, want_z ? hz_of(j - 1) : nullptr);
                 if (rc) return rc;
                 if (sandwich) zdone[j - 1 - jlo] = true;
             }
@@ -2089,9 +2322,16 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
                 if (zdone[j - jlo]) continue;
                 rc = spmm(ctx, RG_SLOT_Z, vm, ptr[j - jlo], zp_of(j), R, plain, nullptr);
                 if (rc) return rc;
+                if (want_z) {
+                    rc = spmm(ctx, RG_SLOT_H, vm, zp_of(j), hz_of(j), R, plain, nullptr);
+                    if (rc) return rc;
+                }
                 zdone[j - jlo] = true;
             }
-            if (k == 0) zpsi0 = zp_of(0);
+            if (k == 0) {
+                zpsi0 = zp_of(0);
+                hzpsi0 = hz_of(0);
+            }
         }
         for (int n = hi; n >= lo; --n) {
             if (ctx->cancel.load()) {
@@ -2109,7 +2349,7 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
             e.qm = ptr[n - 1 - jlo];
             e.cm = (n >= 2) ? ((2 * n - 1 < M) ? 2.0 * gd[2 * n - 1] / R : 0.0) : G1 / R;
             const bool has_next = n + 1 <= N;
-            e.qp = has_next ? ptr[n + 1 - jlo] : zero.p;
+            e.qp = has_next ? ptr[n + 1 - jlo] : zero;
             e.cp = (has_next && 2 * n + 1 < M) ? 2.0 * gd[2 * n + 1] / R : 0.0;
             // lam_n = 2 A~ lam_{n+1} - lam_{n+2} + d_n  (lam_{N+1} = lam_{N+2} = 0)
             rc = apply_chain(ctx, vm, l1, l0, R, e, nullptr);
@@ -2119,10 +2359,15 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
                 rc = accum(l0, ptr[n - 1 - jlo], (n >= 2 ? 2.0 : 1.0) / a);
                 if (rc) return rc;
             } else if (has_next) {
-                // dL/dH = Z (dL/dA) Z^H: the chain just formed Z lam_{n+1}, the
-                // pair (lam_{n+1}, psi_n) enters with coefficient 2
+                // the chain just formed Z lam_{n+1} (vec[3]) and H Z lam_{n+1}
+                // (vec[4]); the pair (lam_{n+1}, psi_n) enters with coefficient 2.
+                // dL/dH = Z (dL/dA) Z^H
                 rc = accum(ctx->vec[3].p, zp_of(n), 2.0 / a);
                 if (rc) return rc;
+                if (want_z) {
+                    rc = accum_z(ctx->vec[4].p, ptr[n - jlo], hz_of(n), l1, 2.0 / a);
+                    if (rc) return rc;
+                }
             }
             void *old = l2;
             l2 = l1;
@@ -2131,21 +2376,50 @@ int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_
         }
     }
     if (sandwich) { // the last pair, (lam_1, psi_0), coefficient 1
-        rc = spmm(ctx, RG_SLOT_Z, vm, l1, zero.p, R, plain, nullptr);
+        rc = spmm(ctx, RG_SLOT_Z, vm, l1, ctx->vec[3].p, R, plain, nullptr);
         if (rc) return rc;
-        rc = accum(zero.p, zpsi0, 1.0 / a);
+        rc = accum(ctx->vec[3].p, zpsi0, 1.0 / a);
+        if (rc) return rc;
+        if (want_z) {
+            rc = spmm(ctx, RG_SLOT_H, vm, ctx->vec[3].p, ctx->vec[4].p, R, plain, nullptr);
+            if (rc) return rc;
+            // psi_0 is stored (the first checkpoint)
+            rc = accum_z(ctx->vec[4].p, stored(0), hzpsi0, l1, 1.0 / a);
+            if (rc) return rc;
+        }
+    }
+    if (grad_h) {
+        rc = download_blocks(ctx, Hs, ctx->bw_grad.p, grad_h);
         if (rc) return rc;
     }
-    std::vector<double> cm((size_t)ng);
-    RG_HIP(ctx, hipMemcpyAsync(cm.data(), grad.p, (size_t)ng * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (grad_z) {
+        rc = download_blocks(ctx, Zs, ctx->bw_gradz.p, grad_z);
+        if (rc) return rc;
+    }
+    if (grad_s) {
+        double *dS = nullptr;
+        rc = patch_backward(ctx, (const double *)ctx->bw_gradz.p, &dS);
+        if (rc) return rc;
+        rc = download_blocks(ctx, ctx->slots[RG_SLOT_S], dS, grad_s);
+        if (rc) return rc;
+    }
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (ctx->prof) drain_events(ctx);
-    const int b = Hs.b;
-    for (int64_t q = 0; q < Hs.nnzb; ++q)
-        for (int r = 0; r < b; ++r)
-            for (int cc = 0; cc < b; ++cc)
-                grad_out[(q * b + r) * b + cc] = cm[(q * b + cc) * b + r];
     return RG_OK;
+}
+
+} // namespace
+
+int rg_kpm_moments_backward(rg_context *ctx, const double *dL_dmu, double *grad_out) {
+    if (!ctx || !dL_dmu || !grad_out) return RG_ERR_INVALID;
+    return kpm_backward(ctx, dL_dmu, grad_out, nullptr, nullptr);
+}
+
+int rg_kpm_moments_backward_ex(rg_context *ctx, const double *dL_dmu, double *grad_h, double *grad_z,
+                               double *grad_s) {
+    if (!ctx || !dL_dmu) return RG_ERR_INVALID;
+    if (!grad_h && !grad_z && !grad_s) return fail(ctx, RG_ERR_INVALID, "no gradient requested");
+    return kpm_backward(ctx, dL_dmu, grad_h, grad_z, grad_s);
 }
 
 int rg_cancel(rg_context *ctx) {

@@ -112,6 +112,34 @@ hipError_t launch_patch_gather(int64_t first, int64_t count, int b, const int64_
 // mode 0: C = 1.5 I - 0.5 A B and resid[p] += ||I - A B||_F^2; mode 1: C = A B
 hipError_t launch_ns_gemm(int mode, int64_t count, int ld, const double *A, const double *B,
                           double *C, double *resid, hipStream_t st);
+// the tiled products plus the tangent modes: 2: C = -0.5 (A B + A2 B2),
+// 3: C = A B + A2 B2
+hipError_t launch_ns_gemm2(int mode, int64_t count, int ld, const double *A, const double *B,
+                           const double *A2, const double *B2, double *C, double *resid,
+                           hipStream_t st);
+// largest patch dimension of the one-wave kernels (16)
+int ns_small_max();
+// patches of dimension <= 16: the whole iteration per patch in one launch;
+// the factor's block rows into zout (column-major blocks on S's pattern),
+// cs / iters / resid per patch
+hipError_t launch_ns_small(int64_t first, int64_t count, int b, const int64_t *row_ptr, const int32_t *col_ind,
+                           const double *svals, double tol, int maxit, double *zout, double *cs, int *iters,
+                           double *resid, hipStream_t st);
+// the tangent of the same iteration seeded with dL/dZ (dz, on S's pattern):
+// dP[p][16][16] -> L_f(A/c, E), f(x) = x^{-1/2}, after `extra` updates past
+// convergence
+hipError_t launch_ns_small_tangent(int64_t first, int64_t count, int b, const int64_t *row_ptr,
+                                   const int32_t *col_ind, const double *svals, const double *dz, double tol,
+                                   int maxit, int extra, double *dP, double *cs, int *iters, double *resid,
+                                   hipStream_t st);
+// tiled tangent seed: Yd = dL/dZ rows of block i, Zd = 0
+hipError_t launch_patch_seed(int64_t first, int64_t count, int b, const int64_t *row_ptr, const int32_t *col_ind,
+                             const double *dz, int ld, double *Yd, double *Zd, hipStream_t st);
+// dS (column-major blocks on S's pattern) += the patch derivatives dP of
+// patches [first, first + count), each scaled by c^{-3/2}
+hipError_t launch_patch_grad_gather(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
+                                    const int32_t *col_ind, int64_t first, int64_t count, int ld,
+                                    const double *dP, const double *cs, double *dS, hipStream_t st);
 hipError_t launch_patch_extract(int64_t first, int64_t count, int b, const int64_t *row_ptr,
                                 const int32_t *col_ind, int ld, const double *Z, const double *cs,
                                 double *out, hipStream_t st);

@@ -374,3 +374,92 @@ def test_recurrence_gradient_matches_dense(oracle):
         fd = (lp_lm[0] - lp_lm[1]) / (2 * h)
         ref = (G[2 * r + p, 2 * cidx + q] + G[2 * cidx + q, 2 * r + p]) / a
         assert abs(fd - ref) < 1e-7
+
+
+# ---------------------------------------------- gradient through S^{-1/2}
+def _spd_overlap(n_blocks, b, rng, scale=0.08):
+    """S = I + scale * (random symmetric block pattern), SPD by construction
+    (diagonally dominant for this scale)."""
+    Hr = random_hermitian_bsr(n_blocks, b, rng, complex_=False)
+    vals = scale * Hr.values.real / np.max(np.abs(Hr.values.real))
+    rows = Hr.block_rows()
+    for s in np.flatnonzero(rows == Hr.col_ind):
+        vals[s] = 0.5 * (vals[s] + vals[s].T) + np.eye(b)
+    return BSR(n_blocks, b, Hr.row_ptr, Hr.col_ind, vals)
+
+
+def _sym_direction(S, rng):
+    """A random direction on S's pattern with D(j, i) = D(i, j)^T."""
+    rows = S.block_rows()
+    D = np.zeros_like(S.values)
+    for s in range(S.values.shape[0]):
+        r, c = int(rows[s]), int(S.col_ind[s])
+        if r > c:
+            continue
+        B = rng.standard_normal(S.values.shape[1:])
+        if r == c:
+            D[s] = 0.5 * (B + B.T)
+        else:
+            s2 = int(np.flatnonzero((rows == c) & (S.col_ind == r))[0])
+            D[s], D[s2] = B, B.T
+    return D
+
+
+@pytest.mark.parametrize("b", [1, 2])
+def test_isqrt_patches_vjp_against_finite_differences(oracle, b):
+    # the Daleckii-Krein patch VJP against central differences of the
+    # eigen-route factor along symmetric pattern-preserving directions
+    rng = np.random.default_rng(11 + b)
+    S = _spd_overlap(9, b, rng)
+    W = rng.standard_normal(S.values.shape)  # L = <W, Z(S)>
+    dS = oracle.isqrt_patches_vjp(S, W)
+
+    def loss(vals):
+        return float(np.sum(W * oracle.isqrt_patches(BSR(S.n_block_rows, b, S.row_ptr, S.col_ind, vals)).values))
+
+    h = 1e-5
+    for _ in range(3):
+        D = _sym_direction(S, rng)
+        fd = (loss(S.values + h * D) - loss(S.values - h * D)) / (2 * h)
+        assert abs(fd - float(np.sum(dS * D))) < 1e-8 * max(1.0, abs(fd))
+    # a planted slip (the transposed derivative operand) is caught
+    bad = oracle.isqrt_patches_vjp(S, np.swapaxes(W, 1, 2)) if b > 1 else -dS
+    D = _sym_direction(S, rng)
+    fd = (loss(S.values + h * D) - loss(S.values - h * D)) / (2 * h)
+    assert abs(fd - float(np.sum(bad * D))) > 1e-6
+
+
+def test_sandwich_gradients_against_finite_differences(oracle):
+    # graphene p_z with onsite, three hopping shells and a nearest-neighbour
+    # overlap: dL/dtheta from the dense chain (H and S gradients contracted
+    # with the affine models) against central differences of the
+    # eigendecomposition moments of Z(S)^T H Z(S)
+    from rsatb_amd.autograd import sk_pz_overlap_parameterization, sk_pz_parameterization
+
+    st = graphene_rect_supercell(3, 3)
+    e = neighbor_edges(st, 2.9)
+    hm = sk_pz_parameterization(st, e, [1.6, 2.5, 2.9])
+    sm = sk_pz_overlap_parameterization(st, neighbor_edges(st, 1.6), [1.6])
+    th_h = np.array([0.1, -2.7, -0.2, -0.18])
+    th_s = np.array([0.1])
+    psi = oracle.samples_rademacher(7, st.n_atoms, 3)
+    M = 16
+    g = np.random.default_rng(2).standard_normal(M)
+    center, a = 0.3, 12.0
+    out = oracle.sandwich_moment_gradients(hm.build(th_h), sm.build(th_s), center, a, psi, g)
+    grad = np.concatenate([hm.contract(out["H"]), sm.contract(out["S"])])
+
+    def loss(th):
+        H, S = hm.build(th[:4]), sm.build(th[4:])
+        Z = oracle.isqrt_patches(S).to_dense()
+        A = (Z.T @ H.to_dense() @ Z - center * np.eye(st.n_atoms)) / a
+        return float(np.sum(g * oracle.dense_moments(A, psi, M).real))
+
+    th = np.concatenate([th_h, th_s])
+    h = 1e-5
+    for k in range(th.size):
+        d = np.zeros(th.size)
+        d[k] = h
+        fd = (loss(th + d) - loss(th - d)) / (2 * h)
+        assert abs(fd - grad[k]) < 1e-7 * max(1.0, abs(fd)), (k, fd, grad[k])
+    assert abs(grad[4]) > 1e-3  # the overlap derivative is not trivially zero
