@@ -516,31 +516,6 @@ def graphene_scale_leg(cells: int, R: int = 16, M: int = 1024) -> dict:
     return out
 
 
-def tblg_transport_model(reps: int):
-    """Twisted bilayer graphene at 1.05 degrees ((31, 32) commensurate cell,
-    11,908 atoms) tiled reps x reps, one p_z orbital per atom, and the two-
-    parameter hopping model H = theta_intra H_intra + theta_inter H_inter:
-    t(r) = Vpp_pi(r) (1 - n^2) + Vpp_sigma(r) n^2 with Vpp_pi = -2.7
-    exp(-(r - 1.42) / 0.319) and Vpp_sigma = 0.48 exp(-(r - 3.35) / 0.319) eV,
-    n the bond's z cosine, on every pair within 3.8 Angstrom; orthogonal
-    basis."""
-    import numpy as np
-
-    from rsatb_amd.autograd import linear_transport_model
-    from rsatb_amd.lattice import neighbor_edges, tile, twisted_bilayer_graphene
-
-    st = tile(twisted_bilayer_graphene(31, 32), (reps, reps, 1))
-    e = neighbor_edges(st, 3.8)
-    inter = st.atom_type[e.src] != st.atom_type[e.dst]
-    n2 = (e.vec[:, 2] / e.dist) ** 2
-    t_r = -2.7 * np.exp(-(e.dist - 1.42) / 0.319) * (1 - n2) + 0.48 * np.exp(-(e.dist - 3.35) / 0.319) * n2
-    ec = np.zeros((e.src.size, 2))
-    ec[~inter, 0] = t_r[~inter]
-    ec[inter, 1] = t_r[inter]
-    model = linear_transport_model(st.n_atoms, e, ec, np.zeros((st.n_atoms, 2)), ("intra", "inter"))
-    return st, e, model
-
-
 def tblg_leg(reps: int, R: int, M: int) -> dict:
     """Differentiable Kubo-Greenwood on twisted bilayer graphene: the
     Greenwood spectrum and its derivative with respect to the interlayer
@@ -556,6 +531,8 @@ def tblg_leg(reps: int, R: int, M: int) -> dict:
     from rsatb_amd.transport import interpolate_spectrum_component
 
     t_setup = time.perf_counter()
+    from rsatb_amd.autograd import tblg_transport_model
+
     st, e, model = tblg_transport_model(reps)
     N = st.n_atoms
     theta = np.array([1.0, 1.0])

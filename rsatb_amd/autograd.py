@@ -155,6 +155,29 @@ def linear_transport_model(n_atoms: int, edges: Edges, edge_coef, onsite_coef, n
     return LinearTransportModel(models[0], tuple(models[1:]))
 
 
+def tblg_transport_model(reps: int):
+    """Twisted bilayer graphene at 1.05 degrees ((31, 32) commensurate cell,
+    11,908 atoms) tiled reps x reps, one p_z orbital per atom, and the two-
+    parameter hopping model H = theta_intra H_intra + theta_inter H_inter:
+    t(r) = Vpp_pi(r) (1 - n^2) + Vpp_sigma(r) n^2 with Vpp_pi = -2.7
+    exp(-(r - 1.42) / 0.319) and Vpp_sigma = 0.48 exp(-(r - 3.35) / 0.319) eV,
+    n the bond's z cosine, on every pair within 3.8 Angstrom; orthogonal
+    basis."""
+    from .lattice import neighbor_edges, tile, twisted_bilayer_graphene
+
+    st = tile(twisted_bilayer_graphene(31, 32), (reps, reps, 1))
+    e = neighbor_edges(st, 3.8)
+    inter = st.atom_type[e.src] != st.atom_type[e.dst]
+    n2 = (e.vec[:, 2] / e.dist) ** 2
+    t_r = -2.7 * np.exp(-(e.dist - 1.42) / 0.319) * (1 - n2) + 0.48 * np.exp(-(e.dist - 3.35) / 0.319) * n2
+    ec = np.zeros((e.src.size, 2))
+    ec[~inter, 0] = t_r[~inter]
+    ec[inter, 1] = t_r[inter]
+    model = linear_transport_model(st.n_atoms, e, ec, np.zeros((st.n_atoms, 2)), ("intra", "inter"))
+    return st, e, model
+
+
+
 class DifferentiableConductivity:
     """The DC spectrum sigma(theta) (dc_spectrum_fftcheb,
     rsatb/calc/fftcheb_transport.hpp:543-653, orthogonal form at Gamma) and

@@ -27,6 +27,7 @@ using namespace rg;
 namespace {
 
 
+
 // Owning device allocation: freed when the owner goes away (a context member
 // at rg_destroy, a local at scope exit, error returns included). Move-only.
 struct DevBuf {
@@ -159,6 +160,12 @@ struct rg_context {
     int patch_maxit = 0;
     // patch adjoint work buffers (kept between calls)
     DevBuf pt_buf[10], pt_cs, pt_it, pt_res, pt_dS;
+
+    // replayed moment loops (small operators): the executable graphs of the
+    // last captured configuration, one per chunk of steps, and its key
+    int graph_mode = 2; // 0 off, 1 always, 2 small operators (RG_GRAPH)
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<uint64_t> graph_key;
 
     bool prof = false;
     KStat stats[RG_KCLASS_COUNT];
@@ -299,7 +306,8 @@ int spmm(rg_context *ctx, int slot, VecMode vm, const void *X, void *Y, int R, E
     return RG_OK;
 }
 
-int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out) {
+int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out, const double2 *partials = nullptr) {
+    if (!partials) partials = static_cast<const double2 *>(ctx->partials.p);
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->prof) {
         a = take_event(ctx);
@@ -307,8 +315,7 @@ int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out) {
         if (!a || !b) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
         RG_HIP(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
     }
-    hipError_t he = launch_reduce(static_cast<const double2 *>(ctx->partials.p), grid, ndot, R,
-                                  out, ctx->stream);
+    hipError_t he = launch_reduce(partials, grid, ndot, R, out, ctx->stream);
     if (he != hipSuccess) return hipfail(ctx, he, "reduce launch");
     if (ctx->prof) {
         RG_HIP(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
@@ -498,6 +505,77 @@ void moments_from_dots(int M, int nraw, bool doubled, Get get, Put put) {
     }
 }
 
+// Chunked capture and replay of the moment loop's launches. A small operator
+// spends more time in launch overhead than in its kernels (graphene 64 x 64:
+// about 1000 launches of a few microseconds per run); its launch sequence is
+// captured once per configuration into HIP graphs of kGraphChunk steps and
+// replayed by later runs with the same key. Cancellation is checked between
+// chunks. No memset is ever captured (copies and kernels only).
+constexpr int kGraphChunk = 64;
+
+void drop_graphs(rg_context *ctx) {
+    for (auto g : ctx->graphs) (void)hipGraphExecDestroy(g);
+    ctx->graphs.clear();
+    ctx->graph_key.clear();
+}
+
+struct Recorder {
+    rg_context *ctx;
+    bool active = false;    // graph mode for this run
+    bool replay = false;    // the cached graphs match: launch them
+    bool capturing = false;
+    bool complete = false; // the whole loop went through this recorder
+    size_t chunk = 0;
+    Recorder(rg_context *c, bool on, const std::vector<uint64_t> &key) : ctx(c), active(on) {
+        if (!active) return;
+        if (ctx->graph_key == key && !ctx->graphs.empty()) {
+            replay = true;
+        } else {
+            drop_graphs(ctx);
+            ctx->graph_key = key;
+        }
+    }
+    ~Recorder() {
+        if (capturing) { // an error inside the capture: close it
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        // a capture cut short (error or cancellation) leaves no partial cache
+        if (active && !replay && !complete) drop_graphs(ctx);
+    }
+    // launches are enqueued (directly or into a capture) unless replaying
+    bool enqueue() const { return !(active && replay); }
+    int begin() {
+        if (!active || replay) return RG_OK;
+        RG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        capturing = true;
+        return RG_OK;
+    }
+    int end() {
+        if (!active) return RG_OK;
+        if (replay) {
+            if (chunk >= ctx->graphs.size()) return fail(ctx, RG_ERR_STATE, "graph cache out of step");
+            RG_HIP(ctx, hipGraphLaunch(ctx->graphs[chunk++], ctx->stream), "hipGraphLaunch");
+            return RG_OK;
+        }
+        hipGraph_t g = nullptr;
+        capturing = false;
+        RG_HIP(ctx, hipStreamEndCapture(ctx->stream, &g), "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            drop_graphs(ctx);
+            return hipfail(ctx, e, "hipGraphInstantiate");
+        }
+        ctx->graphs.push_back(ex);
+        ++chunk;
+        RG_HIP(ctx, hipGraphLaunch(ex, ctx->stream), "hipGraphLaunch");
+        return RG_OK;
+    }
+};
+
 // The moment loop of rg_kpm_moments; with gd, the group-resolved dots of
 // every step are written to gd->out as well (steps as moments_from_dots reads
 // them; nraw_out receives the step count).
@@ -527,15 +605,28 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
         if (rc) return rc;
     }
     const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
-    for (int i = 0; i < 6; ++i) {
-        if ((i == 3 || i == 4) && ctx->form != RG_FORM_SANDWICH) continue;
-        if (i == 5 && doubled) continue;
-        RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
-    }
-    RG_HIP(ctx, ctx->partials.ensure((size_t)spmm_max_grid() * 3 * R * 16), "hipMalloc partials");
     const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
-    RG_HIP(ctx, ctx->raw.ensure((size_t)nsteps * 3 * R * 16), "hipMalloc moment store");
-    double2 *raw = static_cast<double2 *>(ctx->raw.p);
+    // work vectors t0, t1, t2, the sandwich intermediates, the psi copy of the
+    // plain recurrence; the dot partials; the per-step moment store
+    void *wv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *wpart = nullptr, *wraw = nullptr;
+    auto needed = [&](int i) {
+        return !((i == 3 || i == 4) && ctx->form != RG_FORM_SANDWICH) && !(i == 5 && doubled);
+    };
+    const size_t part_bytes = (size_t)spmm_max_grid() * 3 * R * 16, raw_bytes = (size_t)nsteps * 3 * R * 16;
+    {
+        for (int i = 0; i < 6; ++i) {
+            if (!needed(i)) continue;
+            RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
+            wv[i] = ctx->vec[i].p;
+        }
+        RG_HIP(ctx, ctx->partials.ensure(part_bytes), "hipMalloc partials");
+        RG_HIP(ctx, ctx->raw.ensure(raw_bytes), "hipMalloc moment store");
+        wpart = ctx->partials.p;
+        wraw = ctx->raw.p;
+    }
+    double2 *raw = static_cast<double2 *>(wraw);
+    const double2 *parts = static_cast<const double2 *>(wpart);
 
     // forward states kept for the adjoint: psi_j for j <= 1 and j = 0, K-1 (mod K)
     const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32;
@@ -557,33 +648,74 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     // a work buffer that holds neither live state
     auto scratch = [&](const void *x0, const void *x1) -> void * {
         for (int i = 0; i < 3; ++i)
-            if (ctx->vec[i].p != x0 && ctx->vec[i].p != x1) return ctx->vec[i].p;
+            if (wv[i] != x0 && wv[i] != x1) return wv[i];
         return nullptr;
     };
+    // the sandwich's two intermediates
+    void *zx = wv[3], *hzx = wv[4];
 
-    void *t0 = kept(0) ? kept(0) : ctx->vec[0].p;
-    void *t1 = kept(1) ? kept(1) : ctx->vec[1].p;
-    void *t2 = ctx->vec[2].p;
-    rc = stage_samples(ctx, vm, t0);
+    void *t0 = kept(0) ? kept(0) : wv[0];
+    void *t1 = kept(1) ? kept(1) : wv[1];
+    void *t2 = wv[2];
+
+    // graph replay: doubled loops of small operators, outside profiling and
+    // the projected moments
+    const bool small_op = nb <= ((size_t)32 << 20);
+    const bool use_graph = doubled && !gd && !ctx->prof &&
+                           (ctx->graph_mode == 1 || (ctx->graph_mode == 2 && small_op));
+    // The key holds everything a captured launch depends on: the device
+    // addresses and shapes of the operands, the samples and the workspace,
+    // the scalar arguments (window), M and the kernel choice. Values behind
+    // unchanged addresses may change (a re-upload into the same buffers):
+    // the kernels read them at run time.
+    std::vector<uint64_t> key;
+    if (use_graph) {
+        auto bits = [](double x) {
+            uint64_t u;
+            std::memcpy(&u, &x, 8);
+            return u;
+        };
+        auto addr = [](const void *q) { return (uint64_t)(uintptr_t)q; };
+        key = {(uint64_t)M, (uint64_t)precision, (uint64_t)R, (uint64_t)ctx->form, (uint64_t)rows,
+               (uint64_t)vm.complex_vec, (uint64_t)ctx->samples_complex, addr(ctx->psi.p),
+               bits(ctx->center), bits(ctx->half_width), (uint64_t)(keep ? ctx->ckpt_stride : 0),
+               addr(ctx->ckpt.p), addr(wpart), addr(wraw)};
+        for (void *w : wv) key.push_back(addr(w));
+        for (int sl = 0; sl < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++sl) {
+            const Slot &q = ctx->slots[sl];
+            for (uint64_t x : {addr(q.row_ptr.p), addr(q.col_ind.p), addr(vm.fp32 ? q.vals32.p : q.vals64.p),
+                               addr(q.has_xrow ? q.xrow.p : nullptr), (uint64_t)q.nbr, (uint64_t)q.b,
+                               (uint64_t)q.nnzb, (uint64_t)q.is_complex})
+                key.push_back(x);
+        }
+    }
+    Recorder rec(ctx, use_graph, key);
+    const bool enq = rec.enqueue();
+    rc = rec.begin();
     if (rc) return rc;
+    if (enq) {
+        rc = stage_samples(ctx, vm, t0);
+        if (rc) return rc;
+    }
     if (!doubled) {
-        RG_HIP(ctx, hipMemcpyAsync(ctx->vec[5].p, t0, nb, hipMemcpyDeviceToDevice, ctx->stream),
-               "psi copy");
+        RG_HIP(ctx, hipMemcpyAsync(wv[5], t0, nb, hipMemcpyDeviceToDevice, ctx->stream), "psi copy");
     }
 
     const double a = ctx->half_width, c = ctx->center;
     EpiArgs e;
-    e.partials = static_cast<double2 *>(ctx->partials.p);
+    e.partials = static_cast<double2 *>(wpart);
 
     // t1 = A~ t0 with <t0|t0>, <t0|t1>, <t1|t1> (dos.hpp:141-148, 160-163 at n = 1)
     e.mode = EPI_FIRST;
     e.alpha = 1.0 / a;
     e.beta = -c / a;
     int g = 0;
-    rc = apply_chain(ctx, vm, t0, t1, R, e, &g);
-    if (rc) return rc;
-    rc = reduce(ctx, g, 3, R, raw);
-    if (rc) return rc;
+    if (enq) {
+        rc = apply_chain(ctx, vm, t0, t1, R, e, &g, -1, zx, hzx);
+        if (rc) return rc;
+        rc = reduce(ctx, g, 3, R, raw, parts);
+        if (rc) return rc;
+    }
     if (gd) {
         rc = group_dots(ctx, *gd, vm, R, 3, t0, t1, 0);
         if (rc) return rc;
@@ -596,17 +728,27 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
         // t1 = psi_n, t0 = psi_{n-1}; t2 = psi_{n+1} with <t2|t2>, <t2|t1>
         e.mode = EPI_STEP;
         for (int n = 1; 2 * n + 1 < M; ++n) {
-            if (ctx->cancel.load()) {
+            if (use_graph && n % kGraphChunk == 0) {
+                rc = rec.end();
+                if (rc) return rc;
+            }
+            if ((!use_graph || n % kGraphChunk == 0) && ctx->cancel.load()) {
                 (void)hipStreamSynchronize(ctx->stream);
                 drain_events(ctx);
                 return fail(ctx, RG_ERR_CANCELLED, "interrupted");
             }
+            if (use_graph && n % kGraphChunk == 0) {
+                rc = rec.begin();
+                if (rc) return rc;
+            }
             t2 = kept(n + 1) ? kept(n + 1) : scratch(t0, t1);
             e.xp = t0;
-            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
-            if (rc) return rc;
-            rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R);
-            if (rc) return rc;
+            if (enq) {
+                rc = apply_chain(ctx, vm, t1, t2, R, e, &g, -1, zx, hzx);
+                if (rc) return rc;
+                rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R, parts);
+                if (rc) return rc;
+            }
             if (gd) {
                 rc = group_dots(ctx, *gd, vm, R, 2, t2, t1, nraw);
                 if (rc) return rc;
@@ -615,6 +757,9 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
             t0 = t1;
             t1 = t2;
         }
+        rc = rec.end();
+        if (rc) return rc;
+        rec.complete = true;
         if (keep) {
             ctx->ckpt_valid = true;
             ctx->ckpt_M = M;
@@ -627,7 +772,7 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
         }
     } else {
         e.mode = EPI_PLAINSTEP;
-        e.psi = ctx->vec[5].p;
+        e.psi = wv[5];
         for (int m = 2; m < M; ++m) {
             if (ctx->cancel.load()) {
                 (void)hipStreamSynchronize(ctx->stream);
@@ -635,12 +780,12 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
                 return fail(ctx, RG_ERR_CANCELLED, "interrupted");
             }
             e.xp = t0;
-            rc = apply_chain(ctx, vm, t1, t2, R, e, &g);
+            rc = apply_chain(ctx, vm, t1, t2, R, e, &g, -1, zx, hzx);
             if (rc) return rc;
-            rc = reduce(ctx, g, 1, R, raw + (size_t)nraw * 3 * R);
+            rc = reduce(ctx, g, 1, R, raw + (size_t)nraw * 3 * R, parts);
             if (rc) return rc;
             if (gd) {
-                rc = group_dots(ctx, *gd, vm, R, 1, ctx->vec[5].p, t2, nraw);
+                rc = group_dots(ctx, *gd, vm, R, 1, wv[5], t2, nraw);
                 if (rc) return rc;
             }
             ++nraw;
@@ -699,6 +844,11 @@ int rg_create(int device_id, rg_context **out) {
     ctx->device = device_id;
     if (const char *v = std::getenv("RG_XCD_MAP")) set_xcd_map(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_NTCAP")) set_mfma_ntcap(std::atoi(v));
+    if (const char *v = std::getenv("RG_MFMA_GRID")) set_mfma_grid_cap(std::atoi(v));
+    if (const char *v = std::getenv("RG_GRAPH")) {
+        const int m = std::atoi(v);
+        ctx->graph_mode = (m == 0 || m == 1) ? m : 2;
+    }
     if (const char *v = std::getenv("RG_B1_WIDE")) set_b1_wide(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD32")) set_mfma_pd32(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD64")) set_mfma_pd64(std::atoi(v));
@@ -723,6 +873,7 @@ void rg_destroy(rg_context *ctx) {
         (void)hipEventDestroy(pe.stop);
     }
     for (auto ev : ctx->pool) (void)hipEventDestroy(ev);
+    drop_graphs(ctx);
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx; // every device buffer is owned by a DevBuf member and freed here

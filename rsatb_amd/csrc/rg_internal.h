@@ -63,6 +63,9 @@ int spmm_max_grid();
 // XCD-contiguous row ranges in the MFMA SpMM (default on).
 void set_xcd_map(int on);
 void set_mfma_ntcap(int nt);
+// cap on the MFMA SpMM's workgroups (a multiple of 8, <= spmm_max_grid());
+// anything else: the kernel's resident capacity (the default)
+void set_mfma_grid_cap(int g);
 void set_b1_wide(int mode);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
