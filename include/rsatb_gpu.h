@@ -149,6 +149,23 @@ RG_API int rg_bsr_mult(rg_context *ctx, int slot, int R, const double *x, double
 RG_API int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
                    double *mu_cols_out);
 
+/* No counterpart in the reference. rg_kpm_moments for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and the same results
+ * (bit for bit), with the work vectors, dot partials and per-step moment store
+ * in a fixed array of 128 MiB that the library keeps on every device it runs
+ * on until the process ends; the call makes no hipMalloc/hipFree. The caller
+ * still provides the inputs (matrices, window, samples: set beforehand with
+ * the allocating calls) and mu_out / mu_cols_out. RG_ERR_UNSUPPORTED when the
+ * workspace ((3 + 2 sandwich + 1 plain) vectors of rows*R values, 2048*3*R
+ * complex partials and (M/2 + 1)*3*R complex dots, 256-byte aligned) exceeds
+ * the array; RG_ERR_STATE for RG_PREC_FP32 before an FP32 rg_kpm_moments made
+ * the FP32 matrix copies. No forward states are kept (the backward needs
+ * rg_kpm_moments) and no timing events are recorded. All contexts on one
+ * device share the array: calls that overlap on one device race. */
+RG_API int rg_kpm_moments_noalloc(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
+                                  double *mu_cols_out);
+
 /* ---- projected moments (kpm_pdos_moments multivector, pdos.hpp:149-266) ---- */
 
 /* The moments of rg_kpm_moments resolved onto groups of rows from the same

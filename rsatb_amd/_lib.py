@@ -66,6 +66,7 @@ SIGNATURES = {
     "rg_apply_op": (_i, [_vp, _i, _pd, _pd, _i]),
     "rg_bsr_mult": (_i, [_vp, _i, _i, _pd, _pd, _i]),
     "rg_kpm_moments": (_i, [_vp, _i, _i, _i, _pd, _pd]),
+    "rg_kpm_moments_noalloc": (_i, [_vp, _i, _i, _i, _pd, _pd]),
     "rg_kpm_pdos_moments": (_i, [_vp, _i, _i, _i, _i64, _pi32, _i, _pd, _pd]),
     "rg_dos_plan": (_i, [_d, _d, _d, C.POINTER(_i), C.POINTER(_i)]),
     "rg_compute_dos": (_i, [_vp, _d, _d, _d, _i, _pd, _pd]),
@@ -393,6 +394,18 @@ class Context:
             _dptr(cols.view(np.float64)) if per_column else None,
         )
         _check(self._h, rc, "rg_kpm_moments")
+        return (mu, cols) if per_column else mu
+
+    def kpm_moments_noalloc(self, n_moments: int, precision: int = PREC_FP64, doubling: int = DOUBLING_AUTO,
+                            per_column: bool = False):
+        """rg_kpm_moments_noalloc: the same moments with no device allocation."""
+        mu = np.zeros(n_moments, dtype=np.complex128)
+        cols = np.zeros((n_moments, self.R), dtype=np.complex128) if per_column else None
+        rc = self._lib.rg_kpm_moments_noalloc(
+            self._h, int(n_moments), int(precision), int(doubling), _dptr(mu.view(np.float64)),
+            _dptr(cols.view(np.float64)) if per_column else None,
+        )
+        _check(self._h, rc, "rg_kpm_moments_noalloc")
         return (mu, cols) if per_column else mu
 
     def kpm_pdos_moments(self, n_moments: int, group_of_row, n_groups: int, precision: int = PREC_FP64,

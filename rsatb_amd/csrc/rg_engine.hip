@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <complex>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,35 @@ using namespace rg;
 
 namespace {
 
+// Workspace of rg_kpm_moments_noalloc: the moment loop's work vectors, dot
+// partials and moment store, in a file-scope device array that every GPU the
+// library runs on holds until the process ends. Host code never uses the
+// array's name as an address: each device's own address comes from
+// hipGetSymbolAddress (scratch_address).
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+size_t scratch_bytes() { return kScratchBytes; }
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+hipError_t scratch_address(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
 
 
 // Owning device allocation: freed when the owner goes away (a context member
@@ -166,6 +196,11 @@ struct rg_context {
     int graph_mode = 2; // 0 off, 1 always, 2 small operators (RG_GRAPH)
     std::vector<hipGraphExec_t> graphs;
     std::vector<uint64_t> graph_key;
+
+    // this device's no-allocation scratch array (device address), or null
+    // with the reason
+    void *scratch = nullptr;
+    std::string scratch_err;
 
     bool prof = false;
     KStat stats[RG_KCLASS_COUNT];
@@ -580,7 +615,7 @@ struct Recorder {
 // every step are written to gd->out as well (steps as moments_from_dots reads
 // them; nraw_out receives the step count).
 int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
-            double *mu_cols_out, const GroupDots *gd, int *nraw_out = nullptr) {
+            double *mu_cols_out, const GroupDots *gd, int *nraw_out = nullptr, bool noalloc = false) {
     int rc = check_operator(ctx);
     if (rc) return rc;
     if (!ctx->samples_set) return fail(ctx, RG_ERR_STATE, "samples not set (rg_set_samples)");
@@ -601,8 +636,16 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     vm.complex_vec = vec_complex(ctx);
     vm.fp32 = (precision == RG_PREC_FP32);
     if (vm.fp32) {
-        rc = ensure_fp32(ctx);
-        if (rc) return rc;
+        if (noalloc) {
+            for (int i = 0; i < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++i)
+                if (!ctx->slots[i].vals32_valid)
+                    return fail(ctx, RG_ERR_STATE,
+                                "the FP32 copies of the matrices are made by an FP32 rg_kpm_moments call "
+                                "(which allocates); make one before the no-allocation call");
+        } else {
+            rc = ensure_fp32(ctx);
+            if (rc) return rc;
+        }
     }
     const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
     const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
@@ -614,7 +657,30 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
         return !((i == 3 || i == 4) && ctx->form != RG_FORM_SANDWICH) && !(i == 5 && doubled);
     };
     const size_t part_bytes = (size_t)spmm_max_grid() * 3 * R * 16, raw_bytes = (size_t)nsteps * 3 * R * 16;
-    {
+    if (noalloc) {
+        // carved from the scratch array: 256-byte aligned pieces
+        auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        size_t off = 0;
+        size_t offs[8];
+        for (int i = 0; i < 6; ++i) {
+            offs[i] = off;
+            if (needed(i)) off += up(nb);
+        }
+        offs[6] = off;
+        off += up(part_bytes);
+        offs[7] = off;
+        off += up(raw_bytes);
+        if (off > scratch_bytes())
+            return fail(ctx, RG_ERR_UNSUPPORTED,
+                        "the no-allocation moment loop needs " + std::to_string(off) + " bytes of scratch; its array "
+                        "holds " + std::to_string(scratch_bytes()) + " (use rg_kpm_moments)");
+        char *base = static_cast<char *>(ctx->scratch);
+        if (!base) return fail(ctx, RG_ERR_STATE, "no-allocation scratch unavailable: " + ctx->scratch_err);
+        for (int i = 0; i < 6; ++i)
+            if (needed(i)) wv[i] = base + offs[i];
+        wpart = base + offs[6];
+        wraw = base + offs[7];
+    } else {
         for (int i = 0; i < 6; ++i) {
             if (!needed(i)) continue;
             RG_HIP(ctx, ctx->vec[i].ensure(nb), "hipMalloc work vector");
@@ -629,7 +695,7 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     const double2 *parts = static_cast<const double2 *>(wpart);
 
     // forward states kept for the adjoint: psi_j for j <= 1 and j = 0, K-1 (mod K)
-    const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32;
+    const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32 && !noalloc;
     ctx->ckpt_valid = false;
     int Npsi = 1;
     for (int n = 1; 2 * n + 1 < M; ++n) ++Npsi;
@@ -658,10 +724,11 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     void *t1 = kept(1) ? kept(1) : wv[1];
     void *t2 = wv[2];
 
-    // graph replay: doubled loops of small operators, outside profiling and
-    // the projected moments
+    // graph replay: doubled loops of small operators, outside profiling, the
+    // projected moments and the no-allocation entry point (instantiation
+    // allocates)
     const bool small_op = nb <= ((size_t)32 << 20);
-    const bool use_graph = doubled && !gd && !ctx->prof &&
+    const bool use_graph = doubled && !gd && !noalloc && !ctx->prof &&
                            (ctx->graph_mode == 1 || (ctx->graph_mode == 2 && small_op));
     // The key holds everything a captured launch depends on: the device
     // addresses and shapes of the operands, the samples and the workspace,
@@ -860,6 +927,10 @@ int rg_create(int device_id, rg_context **out) {
         delete ctx;
         return RG_ERR_HIP;
     }
+    // the no-allocation entry point's workspace on this device (set-up, once
+    // per device): a failure only disables that entry point
+    const hipError_t se = scratch_address(device_id, &ctx->scratch);
+    if (se != hipSuccess) ctx->scratch_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(se);
     *out = ctx;
     return RG_OK;
 }
@@ -1041,6 +1112,17 @@ int rg_kpm_moments(rg_context *ctx, int M, int precision, int doubling, double *
                    double *mu_cols_out) {
     if (!ctx || !mu_out) return RG_ERR_INVALID;
     return kpm_run(ctx, M, precision, doubling, mu_out, mu_cols_out, nullptr);
+}
+
+int rg_kpm_moments_noalloc(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
+                           double *mu_cols_out) {
+    if (!ctx || !mu_out) return RG_ERR_INVALID;
+    // no timing events either: creating them is an allocation
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    const int rc = kpm_run(ctx, M, precision, doubling, mu_out, mu_cols_out, nullptr, nullptr, true);
+    ctx->prof = prof;
+    return rc;
 }
 
 int rg_kpm_pdos_moments(rg_context *ctx, int M, int precision, int doubling, int64_t n_rows,
