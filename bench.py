@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 steps")
     p.add_argument("--no-legs", action="store_true", help="only the headline")
+    p.add_argument("--leg", default=None,
+                   choices=["c2", "c1", "adjoint", "sk_assembly", "kubo_greenwood", "graphene_scale", "tblg_transport"],
+                   help="run only this secondary leg (no headline) and print its record")
     p.add_argument("--no-c2", action="store_true", help="skip the diamond 8^3 leg")
     p.add_argument("--no-c1", action="store_true", help="skip the graphene 64x64 plain-DOS leg")
     p.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward leg")
@@ -629,6 +632,19 @@ def main() -> int:
     from rsatb_amd.lattice import diamond_supercell
     from rsatb_amd.models import RandomBlockModel
 
+    legs = [("c2", not args.no_c2, lambda: c2_leg(3)),
+            ("c1", not args.no_c1, lambda: c1_leg()),
+            ("adjoint", not args.no_adjoint, lambda: adjoint_leg(args.adj_cells, max(1, min(args.steps, 5)))),
+            ("sk_assembly", not args.no_sk, lambda: sk_leg(args.sk_cells, 3, args.sk_cpu_edges)),
+            ("kubo_greenwood", not args.no_kg,
+             lambda: kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)),
+            ("graphene_scale", not args.no_graphene_scale, lambda: graphene_scale_leg(args.graphene_cells)),
+            ("tblg_transport", not args.no_tblg, lambda: tblg_leg(args.tblg_reps, args.tblg_samples, args.tblg_moments))]
+    if args.leg:
+        fn = dict((k, f) for k, _, f in legs)[args.leg]
+        print(json.dumps({"leg": args.leg, args.leg: fn()}), flush=True)
+        return 0
+
     t_setup = time.perf_counter()
     st = diamond_supercell(args.cells)
     calc = RSATB(st, RandomBlockModel(), SetupOptions(
@@ -721,15 +737,6 @@ def main() -> int:
     ctx.close()
     del calc
     if not args.no_legs:
-        legs = [("c2", not args.no_c2, lambda: c2_leg(3)),
-                ("c1", not args.no_c1, lambda: c1_leg()),
-                ("adjoint", not args.no_adjoint, lambda: adjoint_leg(args.adj_cells, max(1, min(args.steps, 5)))),
-                ("sk_assembly", not args.no_sk, lambda: sk_leg(args.sk_cells, 3, args.sk_cpu_edges)),
-                ("kubo_greenwood", not args.no_kg,
-                 lambda: kg_leg(args.kg_cells, args.kg_samples, args.kg_moments, args.kg_cpu_moments)),
-                ("graphene_scale", not args.no_graphene_scale, lambda: graphene_scale_leg(args.graphene_cells)),
-                ("tblg_transport", not args.no_tblg,
-                 lambda: tblg_leg(args.tblg_reps, args.tblg_samples, args.tblg_moments))]
         for key, on, fn in legs:
             if on:
                 t0 = time.perf_counter()

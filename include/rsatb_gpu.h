@@ -32,6 +32,7 @@
 #ifndef RSATB_GPU_H
 #define RSATB_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -91,7 +92,8 @@ typedef struct rg_context rg_context;
 
 /* arithmetic of the moment loop */
 #define RG_PREC_FP64 0
-#define RG_PREC_FP32 1 /* FP32 SpMM + recurrence, FP64 compensated moment dots */
+#define RG_PREC_FP32 1 /* FP32 SpMM + recurrence; moment dots of exact FP32 products in FP64,
+                                compensated (Neumaier) across workgroup partials */
 
 /* doubling: product-identity two moments per apply (dos.hpp:150-173) */
 #define RG_DOUBLING_AUTO -1 /* on: both provided forms are Hermitian (cheby.hpp:222) */
@@ -108,6 +110,12 @@ typedef struct rg_context rg_context;
 RG_API int rg_create(int device_id, rg_context **ctx);
 RG_API void rg_destroy(rg_context *ctx);
 RG_API const char *rg_last_error(const rg_context *ctx);
+
+/* Free and total device memory of the context's device after its stream has
+ * drained (hipMemGetInfo): for callers that size their problems or watch the
+ * device memory a long fitting loop holds. No counterpart in the reference
+ * (a host-memory code). */
+RG_API int rg_mem_info(rg_context *ctx, size_t *free_bytes, size_t *total_bytes);
 /* Runtime version of this library, for the loader's sanity check. */
 RG_API int rg_version(void);
 

@@ -181,26 +181,30 @@ static void v_axpby(int64_t n, double a, const double *x, double b, double *y) {
     for (int64_t k = 0; k < 2 * n; ++k) y[k] = a * x[k] + b * y[k];
 }
 
-/* dots[c] = sum_i conj(a[i,c]) b[i,c]; fixed-order per-thread partials. */
+/* dots[c] = sum_i conj(a[i,c]) b[i,c]. Each thread sums a static block of
+ * rows into its own slot; the slots are then added in thread order, so the
+ * result is the same on every run with the same thread count. */
 static void v_dot(int64_t rows, int32_t R, const double *a, const double *b, double *dots) {
-    for (int32_t c = 0; c < 2 * R; ++c) dots[c] = 0.0;
-#pragma omp parallel
+    const int nt = omp_get_max_threads();
+    double *part = (double *)calloc((size_t)nt * (size_t)(2 * R), sizeof(double));
+#pragma omp parallel num_threads(nt)
     {
-        double *part = (double *)calloc((size_t)(2 * R), sizeof(double));
+        double *p = part + (size_t)omp_get_thread_num() * (size_t)(2 * R);
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < rows; ++i) {
             const double *ar = a + 2 * i * R, *brw = b + 2 * i * R;
             for (int32_t c = 0; c < R; ++c) {
                 const double xr = ar[2 * c], xi = ar[2 * c + 1];
                 const double yr = brw[2 * c], yi = brw[2 * c + 1];
-                part[2 * c] += xr * yr + xi * yi;
-                part[2 * c + 1] += xr * yi - xi * yr;
+                p[2 * c] += xr * yr + xi * yi;
+                p[2 * c + 1] += xr * yi - xi * yr;
             }
         }
-#pragma omp critical
-        for (int32_t c = 0; c < 2 * R; ++c) dots[c] += part[c];
-        free(part);
     }
+    for (int32_t c = 0; c < 2 * R; ++c) dots[c] = 0.0;
+    for (int t = 0; t < nt; ++t)
+        for (int32_t c = 0; c < 2 * R; ++c) dots[c] += part[(size_t)t * (size_t)(2 * R) + c];
+    free(part);
 }
 
 /* ------------------------------------------------------- apply_form --- */
@@ -240,11 +244,12 @@ static void fused_step(const or_op *op, const double *t0, const double *t1, doub
                        const double *av, int32_t R, double *d22, double *d21) {
     const int64_t rows = op->H->nbr * op->H->b;
     const double alpha = 2.0 / op->half_width, beta = -2.0 * op->center / op->half_width;
-    for (int32_t c = 0; c < 2 * R; ++c) { d22[c] = 0.0; d21[c] = 0.0; }
-#pragma omp parallel
+    const int nt = omp_get_max_threads();
+    double *part = (double *)calloc((size_t)nt * (size_t)(4 * R), sizeof(double));
+#pragma omp parallel num_threads(nt)
     {
-        double *p22 = (double *)calloc((size_t)(2 * R), sizeof(double));
-        double *p21 = (double *)calloc((size_t)(2 * R), sizeof(double));
+        double *p22 = part + (size_t)omp_get_thread_num() * (size_t)(4 * R);
+        double *p21 = p22 + 2 * R;
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < rows; ++i) {
             for (int32_t c = 0; c < R; ++c) {
@@ -258,11 +263,14 @@ static void fused_step(const or_op *op, const double *t0, const double *t1, doub
                 p21[2 * c + 1] += vr * t1[k + 1] - vi * t1[k];
             }
         }
-#pragma omp critical
-        for (int32_t c = 0; c < 2 * R; ++c) { d22[c] += p22[c]; d21[c] += p21[c]; }
-        free(p22);
-        free(p21);
     }
+    /* thread slots added in thread order: the same result on every run */
+    for (int32_t c = 0; c < 2 * R; ++c) { d22[c] = 0.0; d21[c] = 0.0; }
+    for (int t = 0; t < nt; ++t) {
+        const double *p22 = part + (size_t)t * (size_t)(4 * R), *p21 = p22 + 2 * R;
+        for (int32_t c = 0; c < 2 * R; ++c) { d22[c] += p22[c]; d21[c] += p21[c]; }
+    }
+    free(part);
 }
 
 /* kpm_dos_moments, multivector (dos.hpp:124-189).

@@ -59,6 +59,7 @@ SIGNATURES = {
     "rg_create": (_i, [_i, C.POINTER(_vp)]),
     "rg_destroy": (None, [_vp]),
     "rg_last_error": (C.c_char_p, [_vp]),
+    "rg_mem_info": (_i, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "rg_upload_bsr": (_i, [_vp, _i, _i64, _i, _pi64, _pi32, _vp, _i]),
     "rg_set_operator": (_i, [_vp, _i, _d, _d]),
     "rg_set_samples": (_i, [_vp, _i64, _i, _i, _u64, _pd]),
@@ -183,6 +184,12 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def mem_info(self) -> tuple[int, int]:
+        """(free, total) bytes of this context's device (rg_mem_info)."""
+        free, total = C.c_size_t(), C.c_size_t()
+        _check(self._h, self._lib.rg_mem_info(self._h, C.byref(free), C.byref(total)), "rg_mem_info")
+        return int(free.value), int(total.value)
 
     # operands -------------------------------------------------------
     def upload_bsr(self, slot: int, bsr) -> None:

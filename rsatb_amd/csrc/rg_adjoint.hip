@@ -11,21 +11,25 @@ namespace rg {
 namespace {
 
 // G[e] += coef * sum_c Re(U[i, c] conj(V[j, c])) for every stored entry e of
-// the pattern. One group of LPR lanes per point row i (lanes over the sample
-// columns, c = q + t LPR): the row's U values stay in registers while the
-// group walks the row's entries, gathering each V row with coalesced loads
-// and summing the lane partials by xor shuffles. Entries are owned by their
-// point row, so no two groups write the same G element. For b = 1, four
-// entries' V rows are loaded before their reductions.
-template <bool CPLX, int LPR, int UPL, bool B1>
+// the pattern, and with PAIR2 also + coef * sum_c Re(U2[i, c] conj(V2[j, c]))
+// in the same pass. One group of LPR lanes per point row i (lanes over the
+// sample columns, c = q + t LPR): the row's U (and U2) values stay in
+// registers while the group walks the row's entries, gathering each V (V2)
+// row with coalesced loads and summing the lane partials by xor shuffles.
+// Entries are owned by their point row, so no two groups write the same G
+// element. For b = 1, four entries' rows are loaded before their reductions.
+template <bool CPLX, int LPR, int UPL, bool B1, bool PAIR2>
 __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const int64_t *__restrict__ row_ptr,
                                                     const int32_t *__restrict__ col_ind,
                                                     const double *__restrict__ U, const double *__restrict__ V,
+                                                    const double *__restrict__ U2, const double *__restrict__ V2,
                                                     int R, double coef, double *__restrict__ G) {
     using VT = std::conditional_t<CPLX, double2, double>;
     constexpr int NG = 64 / LPR;
     const VT *__restrict__ Uv = reinterpret_cast<const VT *>(U);
     const VT *__restrict__ Vv = reinterpret_cast<const VT *>(V);
+    const VT *__restrict__ Uv2 = reinterpret_cast<const VT *>(U2);
+    const VT *__restrict__ Vv2 = reinterpret_cast<const VT *>(V2);
     const int lane = threadIdx.x & 63;
     const int q = lane % LPR;
     const int64_t nrows = nbr * b;
@@ -47,11 +51,12 @@ __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const in
          i += stride) {
         const int64_t br = B1 ? i : i / b;
         const int rr = B1 ? 0 : (int)(i - br * b);
-        VT u[UPL];
+        VT u[UPL], u2[UPL];
 #pragma unroll
         for (int t = 0; t < UPL; ++t) {
             const int c = q + t * LPR;
             u[t] = c < R ? Uv[i * R + c] : vzero();
+            if constexpr (PAIR2) u2[t] = c < R ? Uv2[i * R + c] : vzero();
         }
         const int64_t s0 = row_ptr[br], s1 = row_ptr[br + 1];
         int64_t s = s0;
@@ -67,6 +72,8 @@ __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const in
                     for (int t = 0; t < UPL; ++t) {
                         const int c = q + t * LPR;
                         if (c < R) acc[k] += dot(u[t], Vv[j * R + c]);
+                        if constexpr (PAIR2)
+                            if (c < R) acc[k] += dot(u2[t], Vv2[j * R + c]);
                     }
                 }
 #pragma unroll
@@ -84,6 +91,8 @@ __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const in
                 for (int t = 0; t < UPL; ++t) {
                     const int c = q + t * LPR;
                     if (c < R) acc += dot(u[t], Vv[(jb + cc) * R + c]);
+                    if constexpr (PAIR2)
+                        if (c < R) acc += dot(u2[t], Vv2[(jb + cc) * R + c]);
                 }
                 acc = reduce(acc);
                 // column-major element (rr, cc) of block s
@@ -128,38 +137,57 @@ int grid_for(int64_t n) {
 
 template <bool CPLX, int LPR, int UPL>
 hipError_t go_grad(int64_t nbr, int b, const int64_t *row_ptr, const int32_t *col_ind, const double *U,
-                   const double *V, int R, double coef, double *G, hipStream_t st) {
+                   const double *V, const double *U2, const double *V2, int R, double coef, double *G,
+                   hipStream_t st) {
     const int64_t rows_per_wg = 4 * (64 / LPR);
     int64_t g = (nbr * b + rows_per_wg - 1) / rows_per_wg;
     if (g > 16384) g = 16384;
     if (g < 1) g = 1;
-    if (b == 1)
-        hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, true>), dim3((unsigned)g), dim3(256), 0, st, nbr, b,
-                           row_ptr, col_ind, U, V, R, coef, G);
-    else
-        hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, false>), dim3((unsigned)g), dim3(256), 0, st, nbr, b,
-                           row_ptr, col_ind, U, V, R, coef, G);
+    const dim3 gd((unsigned)g), bd(256);
+    if (U2) {
+        if (b == 1)
+            hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, true, true>), gd, bd, 0, st, nbr, b, row_ptr, col_ind, U,
+                               V, U2, V2, R, coef, G);
+        else
+            hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, false, true>), gd, bd, 0, st, nbr, b, row_ptr, col_ind,
+                               U, V, U2, V2, R, coef, G);
+    } else {
+        if (b == 1)
+            hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, true, false>), gd, bd, 0, st, nbr, b, row_ptr, col_ind,
+                               U, V, U2, V2, R, coef, G);
+        else
+            hipLaunchKernelGGL((k_grad_accum<CPLX, LPR, UPL, false, false>), gd, bd, 0, st, nbr, b, row_ptr,
+                               col_ind, U, V, U2, V2, R, coef, G);
+    }
     return hipGetLastError();
 }
 
 template <bool CPLX>
 hipError_t grad_by_r(int64_t nbr, int b, const int64_t *row_ptr, const int32_t *col_ind, const double *U,
-                     const double *V, int R, double coef, double *G, hipStream_t st) {
-    if (R <= 8) return go_grad<CPLX, 8, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
-    if (R <= 16) return go_grad<CPLX, 16, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
-    if (R <= 32) return go_grad<CPLX, 32, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
-    if (R <= 64) return go_grad<CPLX, 64, 1>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
-    if (R <= 128) return go_grad<CPLX, 64, 2>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
-    return go_grad<CPLX, 64, 4>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+                     const double *V, const double *U2, const double *V2, int R, double coef, double *G,
+                     hipStream_t st) {
+    if (R <= 8) return go_grad<CPLX, 8, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (R <= 16) return go_grad<CPLX, 16, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (R <= 32) return go_grad<CPLX, 32, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (R <= 64) return go_grad<CPLX, 64, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (R <= 128) return go_grad<CPLX, 64, 2>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    return go_grad<CPLX, 64, 4>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
 }
 
 hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
                              const int32_t *col_ind, const double *U, const double *V, int R,
                              bool cplx, double coef, double *G, hipStream_t st) {
+    return launch_grad_accum2(nbr, b, nnzb, row_ptr, col_ind, U, V, nullptr, nullptr, R, cplx, coef, G, st);
+}
+
+hipError_t launch_grad_accum2(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
+                              const int32_t *col_ind, const double *U, const double *V, const double *U2,
+                              const double *V2, int R, bool cplx, double coef, double *G, hipStream_t st) {
     (void)nnzb;
     if (R < 1 || R > 256) return hipErrorInvalidValue;
-    return cplx ? grad_by_r<true>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st)
-                : grad_by_r<false>(nbr, b, row_ptr, col_ind, U, V, R, coef, G, st);
+    if ((U2 == nullptr) != (V2 == nullptr)) return hipErrorInvalidValue;
+    return cplx ? grad_by_r<true>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st)
+                : grad_by_r<false>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
 }
 
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st) {

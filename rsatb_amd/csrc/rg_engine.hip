@@ -150,6 +150,7 @@ struct rg_context {
 
     DevBuf vec[6]; // t0, t1, t2, tmp1, tmp2, psi copy
     DevBuf partials, raw;
+    DevBuf dpart; // per-step dot partials of a run whose reductions are deferred
     // projected moments: rows sorted by group, chunk and group ranges, dots
     DevBuf pd_rows, pd_chunk, pd_grp, pd_partial, pd_out;
     // Slater-Koster assembly scratch (kept between calls): table, atoms,
@@ -357,6 +358,28 @@ int reduce(rg_context *ctx, int grid, int ndot, int R, double2 *out, const doubl
         ctx->pending.push_back({a, b, RG_KCLASS_REDUCE});
         ctx->stats[RG_KCLASS_REDUCE].bytes = (double)grid * ndot * R * 16.0 + ndot * R * 16.0;
         ctx->stats[RG_KCLASS_REDUCE].flops = (double)grid * ndot * R * 2.0;
+    }
+    return RG_OK;
+}
+
+// Reductions of nsteps steps in one launch (partials and out advance by
+// their strides), timed as one launch of the reduction class.
+int reduce_steps(rg_context *ctx, int grid, int ndot, int R, double2 *out, int64_t ostride,
+                 const double2 *partials, int64_t pstride, int nsteps) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (!a || !b) return fail(ctx, RG_ERR_HIP, "hipEventCreate failed");
+        RG_HIP(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
+    }
+    hipError_t he = launch_reduce_steps(partials, pstride, grid, ndot, R, out, ostride, nsteps, ctx->stream);
+    if (he != hipSuccess) return hipfail(ctx, he, "reduce launch");
+    if (ctx->prof) {
+        RG_HIP(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
+        ctx->pending.push_back({a, b, RG_KCLASS_REDUCE});
+        ctx->stats[RG_KCLASS_REDUCE].bytes = (double)grid * ndot * R * 16.0 * nsteps + ndot * R * 16.0 * nsteps;
+        ctx->stats[RG_KCLASS_REDUCE].flops = (double)grid * ndot * R * 2.0 * nsteps;
     }
     return RG_OK;
 }
@@ -694,6 +717,22 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     double2 *raw = static_cast<double2 *>(wraw);
     const double2 *parts = static_cast<const double2 *>(wpart);
 
+    // Deferred reductions (small operators): a step's kernels take a few
+    // microseconds, of which a separate reduction launch per step is half.
+    // Each doubled step then writes its dot partials to a region of its own,
+    // and one batched launch after the loop reduces them all, in the same
+    // fixed order (the moments are the same bit for bit).
+    const int chain_slot = ctx->form == RG_FORM_SANDWICH ? RG_SLOT_ZH : RG_SLOT_H;
+    const int g_step = spmm_launch_grid(dev_view(ctx->slots[chain_slot], vm.fp32), vm, R);
+    const int64_t pstride = (int64_t)g_step * 2 * R; // double2 per step region
+    const bool defer = doubled && !gd && !noalloc && nb <= ((size_t)32 << 20) && nsteps > 2 &&
+                       (size_t)nsteps * (size_t)pstride * 16 <= ((size_t)512 << 20);
+    double2 *dparts = nullptr;
+    if (defer) {
+        RG_HIP(ctx, ctx->dpart.ensure((size_t)nsteps * (size_t)pstride * 16), "hipMalloc step partials");
+        dparts = static_cast<double2 *>(ctx->dpart.p);
+    }
+
     // forward states kept for the adjoint: psi_j for j <= 1 and j = 0, K-1 (mod K)
     const bool keep = doubled && ctx->ckpt_stride > 0 && !vm.fp32 && !noalloc;
     ctx->ckpt_valid = false;
@@ -746,7 +785,7 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
         key = {(uint64_t)M, (uint64_t)precision, (uint64_t)R, (uint64_t)ctx->form, (uint64_t)rows,
                (uint64_t)vm.complex_vec, (uint64_t)ctx->samples_complex, addr(ctx->psi.p),
                bits(ctx->center), bits(ctx->half_width), (uint64_t)(keep ? ctx->ckpt_stride : 0),
-               addr(ctx->ckpt.p), addr(wpart), addr(wraw)};
+               addr(ctx->ckpt.p), addr(wpart), addr(wraw), addr(dparts), (uint64_t)pstride};
         for (void *w : wv) key.push_back(addr(w));
         for (int sl = 0; sl < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++sl) {
             const Slot &q = ctx->slots[sl];
@@ -810,11 +849,15 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
             }
             t2 = kept(n + 1) ? kept(n + 1) : scratch(t0, t1);
             e.xp = t0;
+            if (defer) e.partials = dparts + (size_t)nraw * (size_t)pstride;
             if (enq) {
                 rc = apply_chain(ctx, vm, t1, t2, R, e, &g, -1, zx, hzx);
                 if (rc) return rc;
-                rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R, parts);
-                if (rc) return rc;
+                if (defer && g != g_step) return fail(ctx, RG_ERR_STATE, "step grid differs from its plan");
+                if (!defer) {
+                    rc = reduce(ctx, g, 2, R, raw + (size_t)nraw * 3 * R, parts);
+                    if (rc) return rc;
+                }
             }
             if (gd) {
                 rc = group_dots(ctx, *gd, vm, R, 2, t2, t1, nraw);
@@ -823,6 +866,11 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
             ++nraw;
             t0 = t1;
             t1 = t2;
+        }
+        if (defer && enq && nraw > 1) {
+            rc = reduce_steps(ctx, g_step, 2, R, raw + (size_t)3 * R, (int64_t)3 * R, dparts + pstride, pstride,
+                              nraw - 1);
+            if (rc) return rc;
         }
         rc = rec.end();
         if (rc) return rc;
@@ -951,6 +999,14 @@ void rg_destroy(rg_context *ctx) {
 }
 
 const char *rg_last_error(const rg_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rg_mem_info(rg_context *ctx, size_t *free_bytes, size_t *total_bytes) {
+    if (!ctx || !free_bytes || !total_bytes) return RG_ERR_INVALID;
+    RG_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    RG_HIP(ctx, hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+    return RG_OK;
+}
 
 int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *row_ptr,
                   const int32_t *col_ind, const void *values, int dtype) {
@@ -2514,11 +2570,15 @@ int kpm_backward(rg_context *ctx, const double *dL_dmu, double *grad_h, double *
     };
     auto accum = [&](const void *u, const void *v, double coef) { return accum_on(Hs, ctx->bw_grad.p, u, v, coef); };
     // dL/dZ = (H Z G + H Z G^T) / a on Z's pattern for the pair (lam, psi)
-    // of dL/dA~ = G = sum coef lam psi^T (Z^H tied to Z^T)
+    // of dL/dA~ = G = sum coef lam psi^T (Z^H tied to Z^T): both terms in one
+    // pass over the pattern
     auto accum_z = [&](const void *hzl, const void *psi, const void *hzp, const void *lam, double coef) -> int {
-        int r2 = accum_on(Zs, ctx->bw_gradz.p, hzl, psi, coef);
-        if (r2) return r2;
-        return accum_on(Zs, ctx->bw_gradz.p, hzp, lam, coef);
+        RG_HIP(ctx, launch_grad_accum2(Zs.nbr, Zs.b, Zs.nnzb, (const int64_t *)Zs.row_ptr.p,
+                                       (const int32_t *)Zs.col_ind.p, (const double *)hzl, (const double *)psi,
+                                       (const double *)hzp, (const double *)lam, R, vm.complex_vec, coef,
+                                       (double *)ctx->bw_gradz.p, ctx->stream),
+               "gradient accumulation");
+        return RG_OK;
     };
     const int nseg = N / K;
     void *zpsi0 = nullptr, *hzpsi0 = nullptr;

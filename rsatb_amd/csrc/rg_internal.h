@@ -64,7 +64,7 @@ int spmm_max_grid();
 void set_xcd_map(int on);
 void set_mfma_ntcap(int nt);
 // cap on the MFMA SpMM's workgroups (a multiple of 8, <= spmm_max_grid());
-// anything else: the kernel's resident capacity (the default)
+// anything else: two workgroups per compute unit (the default)
 void set_mfma_grid_cap(int g);
 void set_b1_wide(int mode);
 void set_mfma_pd32(int p);
@@ -75,6 +75,10 @@ void set_mfma_pd64(int p);
 hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
                                   hipStream_t stream);
 
+// The grid (workgroups, so the number of dot partials) launch_spmm uses
+// for this operand, vector storage and R.
+int spmm_launch_grid(const BsrDev &A, VecMode vm, int R);
+
 // Y = epilogue(A X). X, Y: [nbr*b][R] in the storage `vm` describes.
 hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int R,
                        const EpiArgs &e, hipStream_t stream, int *grid_out);
@@ -82,6 +86,11 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
 // out[d*R + u] = sum_g partials[(g*ndot + d)*R + u], fixed order.
 hipError_t launch_reduce(const double2 *partials, int ngroups, int ndot, int R, double2 *out,
                          hipStream_t stream);
+
+// The same for nsteps steps in one launch: step s reduces partials +
+// s pstride into out + s ostride (double2 strides).
+hipError_t launch_reduce_steps(const double2 *partials, int64_t pstride, int ngroups, int ndot, int R,
+                               double2 *out, int64_t ostride, int nsteps, hipStream_t stream);
 
 // Sample block: Rademacher (real or complex with zero imaginary part) or
 // random phase (complex). scale = 1/sqrt(rows) computed on the host.
@@ -164,6 +173,11 @@ hipError_t launch_lanczos_scale(int64_t rows, int R, bool cplx, const double *w,
 hipError_t launch_grad_accum(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
                              const int32_t *col_ind, const double *U, const double *V, int R,
                              bool cplx, double coef, double *G, hipStream_t st);
+// the same with a second pair in one pass: G[e] += coef * sum_c (Re(U[i, c]
+// conj(V[j, c])) + Re(U2[i, c] conj(V2[j, c])))
+hipError_t launch_grad_accum2(int64_t nbr, int b, int64_t nnzb, const int64_t *row_ptr,
+                              const int32_t *col_ind, const double *U, const double *V, const double *U2,
+                              const double *V2, int R, bool cplx, double coef, double *G, hipStream_t st);
 // y = 0 over n doubles
 hipError_t launch_zero(double *y, int64_t n, hipStream_t st);
 // out[c * rows + r] = (double) in[c * ld_in + r] for r < rows, c < cols

@@ -39,13 +39,14 @@ constexpr int kMaxGrid = 2048;
 int g_xcd_map = 1;
 // Upper bound on the 16-column tiles one wave owns in the MFMA SpMM.
 int g_mfma_ntcap = 16;
-// Cap on the MFMA SpMM grid (workgroups, at most kMaxGrid); 0: the kernel's
-// resident capacity (occupancy x compute units). With no more workgroups than
-// fit at once, the grid-stride loop keeps each XCD's rows in flight a compact
-// window of consecutive block rows; a larger grid queues workgroups whose rows
-// lie far ahead. Measured at the diamond 24^3 sandwich (R = 128 complex):
-// FP64 plain product 4994 -> 4680 us, FP32 2886 -> 2579 us going from 2048 to
-// 512 workgroups; diamond 8^3 FP64 plain 93.9 -> 87.4 us.
+// Cap on the MFMA SpMM grid (workgroups, at most kMaxGrid); 0: two
+// workgroups per compute unit. A grid-stride loop over fewer workgroups keeps
+// the block rows in flight on each XCD closer together than the 2048-workgroup
+// grid that covers every row once. Measured at the diamond 24^3 sandwich
+// (R = 128 complex; the FP64 kernels hold one workgroup per CU, the FP32 plain
+// product three), 2048 / resident / 512 / 256 workgroups: FP64 plain product
+// 4994 / 4807 / 4638 / 4808 us, FP32 plain 2886 / 2870 / 2560 / 2709 us;
+// diamond 8^3 FP64 plain 93.9 / 92.8 / 87.5 / 102.4 us.
 int g_mfma_grid_cap = 0;
 // b = 1 SpMM over real vectors with 2 columns per lane: 1 always, 0 never,
 // 2 by precision and row length.
@@ -306,6 +307,20 @@ int lpr_for(int R) { return R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64; }
 
 int upl_for(int R) { return R <= 64 ? 1 : R <= 128 ? 2 : 4; }
 
+// The wide b = 1 layout (half the lanes per row, two columns per lane) for
+// scalar entries and real vectors, 8 < R <= 32: FP32, or rows of >= 8
+// entries (g_b1_wide 2), always (1) or never (0).
+bool b1_wide(const BsrDev &A, bool fp32, bool cv, int R) {
+    if (A.b != 1 || cv || R <= 8 || R > 32) return false;
+    return g_b1_wide == 1 || (g_b1_wide == 2 && (fp32 || A.nnzb >= 8 * A.nbr));
+}
+
+// Lanes per block row of the VALU SpMM launch for this operand.
+int valu_lpr(const BsrDev &A, bool fp32, bool cv, int R) {
+    if (b1_wide(A, fp32, cv, R)) return R <= 16 ? 8 : 16;
+    return lpr_for(R);
+}
+
 template <typename S, bool CM, bool CV, int BS, int LPR, int UPL>
 hipError_t go(const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, int grid,
               hipStream_t st) {
@@ -324,10 +339,9 @@ hipError_t by_layout(const BsrDev &A, const void *X, void *Y, int R, const EpiAr
         // at 1,048,576 rows of 4 entries, FP64 86 -> 75 us at 190,528 rows of
         // 22 entries, but FP64 163 -> 193 us at 4 entries a row; complex
         // vectors (R = 32) slower. Automatic: FP32, or rows of >= 8 entries.
-        const bool wide = g_b1_wide == 1 ||
-                          (g_b1_wide == 2 && (sizeof(S) == 4 || A.nnzb >= 8 * A.nbr));
-        if (wide && R > 8 && R <= 16) return go<S, CM, CV, BS, 8, 2>(A, X, Y, R, e, grid, st);
-        if (wide && R > 16 && R <= 32) return go<S, CM, CV, BS, 16, 2>(A, X, Y, R, e, grid, st);
+        const bool wide = b1_wide(A, sizeof(S) == 4, CV, R);
+        if (wide && R <= 16) return go<S, CM, CV, BS, 8, 2>(A, X, Y, R, e, grid, st);
+        if (wide) return go<S, CM, CV, BS, 16, 2>(A, X, Y, R, e, grid, st);
     }
     switch (lpr_for(R)) {
     case 8: return go<S, CM, CV, BS, 8, 1>(A, X, Y, R, e, grid, st);
@@ -666,26 +680,25 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
 }
 
-// Workgroups of `kernel` resident at once on the current device, rounded
-// down to whole XCD rounds (a multiple of 8), at most kMaxGrid.
-template <typename K> int resident_wgs(K kernel) {
-    int per_cu = 0, dev = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, 0) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1)
+// Two workgroups per compute unit of the current device, a multiple of 8
+// (whole XCD rounds), at most kMaxGrid.
+int mfma_default_grid() {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
         return kMaxGrid;
-    const int64_t g = ((int64_t)per_cu * ncu / 8) * 8;
+    const int64_t g = ((int64_t)2 * ncu / 8) * 8;
     return (int)std::max<int64_t>(8, std::min<int64_t>(g, kMaxGrid));
 }
 
-template <typename S, int NT, int MODE, bool CV, int PD> int mfma_grid(int grid_req) {
-    static std::atomic<int> cap{0}; // per kernel instantiation
+int mfma_grid(int grid_req) {
+    static std::atomic<int> def{0};
     int c = g_mfma_grid_cap;
     if (c <= 0) {
-        c = cap.load(std::memory_order_relaxed);
+        c = def.load(std::memory_order_relaxed);
         if (c == 0) {
-            c = resident_wgs(k_spmm_mfma<S, NT, MODE, CV, PD>);
-            cap.store(c, std::memory_order_relaxed);
+            c = mfma_default_grid();
+            def.store(c, std::memory_order_relaxed);
         }
     }
     return grid_req < c ? grid_req : c;
@@ -694,14 +707,7 @@ template <typename S, int NT, int MODE, bool CV, int PD> int mfma_grid(int grid_
 template <typename S, int NT, bool CV, int PD>
 hipError_t go_mfma_pd(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                       int grid_req, hipStream_t st, int *grid_out) {
-    int grid = grid_req;
-    switch (e.mode) {
-    case EPI_PLAIN: grid = mfma_grid<S, NT, EPI_PLAIN, CV, PD>(grid_req); break;
-    case EPI_FIRST: grid = mfma_grid<S, NT, EPI_FIRST, CV, PD>(grid_req); break;
-    case EPI_STEP: grid = mfma_grid<S, NT, EPI_STEP, CV, PD>(grid_req); break;
-    case EPI_PLAINSTEP: grid = mfma_grid<S, NT, EPI_PLAINSTEP, CV, PD>(grid_req); break;
-    default: grid = mfma_grid<S, NT, EPI_ADJ, CV, PD>(grid_req); break;
-    }
+    const int grid = mfma_grid(grid_req);
     if (grid_out) *grid_out = grid;
     const int xcd = g_xcd_map;
     const dim3 gd(grid), bd(64 * kWaves);
@@ -767,8 +773,13 @@ __device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
     s = t;
 }
 
+// One (dot, column) pair per wave; blockIdx.y selects a step of a batched
+// reduction (partials and out advance by their step strides).
 __global__ __launch_bounds__(256) void k_reduce(const double2 *__restrict__ partials, int ngroups,
-                                                int ndot, int R, double2 *__restrict__ out) {
+                                                int ndot, int R, double2 *__restrict__ out, int64_t pstride,
+                                                int64_t ostride) {
+    partials += (int64_t)blockIdx.y * pstride;
+    out += (int64_t)blockIdx.y * ostride;
     const int lane = threadIdx.x & 63;
     const int pair = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const int npairs = ndot * R;
@@ -897,6 +908,18 @@ int grid_1d(int64_t n) {
 
 } // namespace
 
+namespace {
+// Workgroups of the VALU SpMM: kWaves waves of 64 / lpr block rows each.
+int valu_grid(int64_t nbr, int lpr) {
+    const int G = 64 / lpr;
+    const int64_t rows_per_wg = (int64_t)kWaves * G;
+    int64_t g = (nbr + rows_per_wg - 1) / rows_per_wg;
+    if (g > kMaxGrid) g = kMaxGrid;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+} // namespace
+
 int spmm_grid(int64_t nbr, int R) {
     const int G = 64 / lpr_for(R);
     const int64_t rows_per_wg = (int64_t)kWaves * G;
@@ -904,6 +927,20 @@ int spmm_grid(int64_t nbr, int R) {
     if (g > kMaxGrid) g = kMaxGrid;
     if (g < 1) g = 1;
     return (int)g;
+}
+
+int spmm_launch_grid(const BsrDev &A, VecMode vm, int R) {
+    const int W = R * (vm.complex_vec ? 2 : 1);
+    int nt = mfma_nt(W);
+    while (nt > 2 && nt > g_mfma_ntcap) nt /= 2;
+    const int nchunks = nt > 0 ? W / (16 * nt) : 0;
+    if (!A.is_complex && A.b >= 8 && A.xrow && nt > 0 && kWaves % nchunks == 0) {
+        int64_t g = (A.nbr * nchunks + kWaves - 1) / kWaves;
+        if (g > kMaxGrid) g = kMaxGrid;
+        g = ((g + 7) / 8) * 8;
+        return mfma_grid((int)g);
+    }
+    return valu_grid(A.nbr, valu_lpr(A, vm.fp32, vm.complex_vec, R));
 }
 
 hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int R,
@@ -918,7 +955,7 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
         int64_t g = (A.nbr * nchunks + kWaves - 1) / kWaves;
         if (g > kMaxGrid) g = kMaxGrid;
         g = ((g + 7) / 8) * 8; // whole XCD rounds (kMaxGrid is a multiple of 8)
-        const int grid = (int)g; // capped further per kernel (mfma_grid)
+        const int grid = (int)g; // capped further by mfma_grid
         if (vm.fp32) {
             return vm.complex_vec ? by_nt<float, true>(nt, A, X, Y, W, R, e, grid, stream, grid_out)
                                   : by_nt<float, false>(nt, A, X, Y, W, R, e, grid, stream, grid_out);
@@ -926,7 +963,9 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
         return vm.complex_vec ? by_nt<double, true>(nt, A, X, Y, W, R, e, grid, stream, grid_out)
                               : by_nt<double, false>(nt, A, X, Y, W, R, e, grid, stream, grid_out);
     }
-    const int grid = spmm_grid(A.nbr, R);
+    // sized by the layout the launch takes (the wide b = 1 layout covers twice
+    // the rows per workgroup)
+    const int grid = valu_grid(A.nbr, valu_lpr(A, vm.fp32, vm.complex_vec, R));
     if (grid_out) *grid_out = grid;
     if (!vm.fp32) {
         if (A.is_complex) return by_block<double, true, true>(A, X, Y, R, e, grid, stream);
@@ -956,10 +995,16 @@ hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, i
 
 hipError_t launch_reduce(const double2 *partials, int ngroups, int ndot, int R, double2 *out,
                          hipStream_t stream) {
+    return launch_reduce_steps(partials, 0, ngroups, ndot, R, out, 0, 1, stream);
+}
+
+hipError_t launch_reduce_steps(const double2 *partials, int64_t pstride, int ngroups, int ndot, int R,
+                               double2 *out, int64_t ostride, int nsteps, hipStream_t stream) {
+    if (nsteps < 1) return hipSuccess;
     const int npairs = ndot * R;
     const int grid = (npairs + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(64 * kWaves), 0, stream, partials, ngroups, ndot,
-                       R, out);
+    hipLaunchKernelGGL(k_reduce, dim3(grid, (unsigned)nsteps), dim3(64 * kWaves), 0, stream, partials, ngroups,
+                       ndot, R, out, pstride, ostride);
     return hipGetLastError();
 }
 

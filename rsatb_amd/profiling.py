@@ -127,19 +127,62 @@ def render(summary, title: str, commands: list[str]) -> str:
     return "\n".join(lines) + "\n"
 
 
+def read_pmc_all(d: Path):
+    """{kernel: {counter: mean value per dispatch}} of one --pmc run."""
+    acc = defaultdict(lambda: defaultdict(list))
+    for r in _csv(_find(d, "counter_collection.csv")):
+        acc[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+def render_counters(d: Path, title: str, commands: list[str], simds: int, kernels: list[str],
+                    grbm_instances: int = 8) -> str:
+    """Per-kernel L2 hit rate (TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum)) and
+    MFMA utilisation (SQ_VALU_MFMA_BUSY_CYCLES / (active cycles x SIMDs)); the
+    CSV's GRBM_GUI_ACTIVE is summed over the XCDs' instances, so the active
+    cycles are GRBM_GUI_ACTIVE / grbm_instances."""
+    pm = read_pmc_all(d)
+    lines = [f"# {title}", "", "Command (from the repository root on the GPU box; output under gpurun_out/):", ""]
+    lines += [f"    {cmd}" for cmd in commands]
+    lines += ["", f"MFMA utilisation = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / {grbm_instances} XCDs x "
+              f"{simds} SIMDs); "
+              "L2 hit rate = TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum); means per dispatch.", "",
+              "| kernel | L2 hit rate | L2 misses per dispatch | MFMA utilisation |",
+              "|---|---|---|---|"]
+    for k in kernels:
+        c = pm.get(k)
+        if not c:
+            continue
+        hit, miss = c.get("TCC_HIT_sum", 0.0), c.get("TCC_MISS_sum", 0.0)
+        busy, active = c.get("SQ_VALU_MFMA_BUSY_CYCLES"), c.get("GRBM_GUI_ACTIVE")
+        hr = f"{hit / (hit + miss):.3f}" if hit + miss > 0 else "-"
+        mu = f"{busy / (active / grbm_instances * simds):.3f}" if busy is not None and active else "-"
+        lines.append(f"| `{k}` | {hr} | {miss:.3e} | {mu} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--trace", required=True, type=Path)
-    ap.add_argument("--fetch", required=True, type=Path)
-    ap.add_argument("--write", required=True, type=Path)
-    ap.add_argument("--copy-bytes", required=True, type=float)
+    ap.add_argument("--counters", type=Path, default=None,
+                    help="summarize one --pmc run of TCC_HIT_sum TCC_MISS_sum SQ_VALU_MFMA_BUSY_CYCLES "
+                         "GRBM_GUI_ACTIVE instead (with --simds, --kernel)")
+    ap.add_argument("--simds", type=int, default=1024)
+    ap.add_argument("--grbm-instances", type=int, default=8, help="XCDs the GRBM_GUI_ACTIVE value is summed over")
+    ap.add_argument("--kernel", action="append", default=[])
+    ap.add_argument("--trace", type=Path)
+    ap.add_argument("--fetch", type=Path)
+    ap.add_argument("--write", type=Path)
+    ap.add_argument("--copy-bytes", type=float)
     ap.add_argument("--copy-write-bytes", type=float, default=None)
     ap.add_argument("--title", required=True)
-    ap.add_argument("--tag", required=True, help="workload tag the benchmark matches, e.g. diamond8_R64_sandwich")
+    ap.add_argument("--tag", default="", help="workload tag the benchmark matches, e.g. diamond8_R64_sandwich")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--out", required=True, type=Path)
-    ap.add_argument("--json", required=True, type=Path)
+    ap.add_argument("--json", type=Path)
     a = ap.parse_args(argv)
+    if a.counters is not None:
+        a.out.write_text(render_counters(a.counters, a.title, a.command, a.simds, a.kernel, a.grbm_instances))
+        return
     s = summarize(a.trace, a.fetch, a.write, a.copy_bytes, a.copy_write_bytes)
     a.out.write_text(render(s, a.title, a.command))
     s["title"] = a.title

@@ -229,17 +229,6 @@ def test_stale_forward_states_are_rejected():
         ctx.close()
 
 
-def _device_free_bytes() -> int:
-    # hipMemGetInfo of the current device (the HIP runtime the library uses)
-    import ctypes
-
-    hip = ctypes.CDLL("libamdhip64.so")
-    free, total = ctypes.c_size_t(), ctypes.c_size_t()
-    assert hip.hipDeviceSynchronize() == 0
-    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
-    return int(free.value)
-
-
 def test_repeated_forward_backward_holds_device_memory():
     # 50 forward + backward calls (patches rebuilt each time) on a C4-shaped
     # graph: device free memory returns to its level after the first call
@@ -250,11 +239,11 @@ def test_repeated_forward_backward_holds_device_memory():
         w = np.linspace(1.0, -1.0, 64)
         eng.forward(theta)
         eng.backward(w)
-        free0 = _device_free_bytes()
+        free0, _ = eng.ctx.mem_info()
         for _ in range(50):
             eng.forward(theta)
             eng.backward(w)
-        free1 = _device_free_bytes()
+        free1, _ = eng.ctx.mem_info()
     finally:
         eng.ctx.close()
     print(f"\nfree memory change over 50 calls: {(free1 - free0) / 2**20:.3f} MiB", flush=True)
