@@ -248,6 +248,17 @@ RG_API int rg_compute_dos(rg_context *ctx, double emin, double emax, double de, 
 RG_API int rg_kpm_density(const double *mu, int M, const double *x, int64_t nx, double edge_inset,
                    int kernel, double *rho_out);
 
+/* The MFMA SpMM plan the library derives for a square block pattern (real
+ * slots with b >= 8 get one at every write): sched_out[nbr] the block rows
+ * in processing order (regions of *sreg_out rows, one XCD's rows in flight,
+ * each grown breadth-first over the block graph), kperm_out[nnzb] the stored
+ * block every position of a row takes (blocks ordered by a distance-2
+ * colouring of the block columns). Host only, needs no device. No
+ * counterpart in the reference (an execution order internal to this
+ * library); for inspection and tests. */
+RG_API int rg_mfma_plan(int64_t n_block_rows, const int64_t *row_ptr, const int32_t *col_ind,
+                        int32_t *sched_out, int32_t *kperm_out, int32_t *sreg_out);
+
 /* ---- operand inspection (dump_operator_coo, rsatb/pybind_rsatb.cpp:612-670) ---- */
 
 RG_API int rg_bsr_info(rg_context *ctx, int slot, int64_t *n_block_rows, int *block_dim,

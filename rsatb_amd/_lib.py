@@ -72,6 +72,7 @@ SIGNATURES = {
     "rg_dos_plan": (_i, [_d, _d, _d, C.POINTER(_i), C.POINTER(_i)]),
     "rg_compute_dos": (_i, [_vp, _d, _d, _d, _i, _pd, _pd]),
     "rg_kpm_density": (_i, [_pd, _i, _pd, _i64, _d, _i, _pd]),
+    "rg_mfma_plan": (_i, [_i64, _pi64, _pi32, _pi32, _pi32, C.POINTER(C.c_int32)]),
     "rg_bsr_info": (_i, [_vp, _i, _pi64, C.POINTER(_i), _pi64, C.POINTER(_i)]),
     "rg_download_bsr": (_i, [_vp, _i, _pi64, _pi32, _pd]),
     "rg_sk_assemble": (_i, [_vp, _i, C.POINTER(SkTableC), _i64, _pi32, _pd, _pd, _i64, _pi32, _pi32, _pd,
@@ -152,6 +153,23 @@ def kpm_density(mu: np.ndarray, x: np.ndarray, edge_inset: float, kernel: int = 
     if rc != RG_OK:
         raise RsatbGpuError(rc, "kpm_density: bad arguments")
     return rho
+
+
+def mfma_plan(row_ptr: np.ndarray, col_ind: np.ndarray) -> tuple[np.ndarray, np.ndarray, int]:
+    """The MFMA SpMM plan of a square block pattern (rg_mfma_plan): block rows
+    in processing order, the stored block behind every position, and the
+    region size. Host only."""
+    rp = np.ascontiguousarray(np.asarray(row_ptr, dtype=np.int64))
+    ci = np.ascontiguousarray(np.asarray(col_ind, dtype=np.int32))
+    n = int(rp.shape[0]) - 1
+    sched = np.zeros(max(n, 0), dtype=np.int32)
+    kperm = np.zeros(ci.shape[0], dtype=np.int32)
+    sreg = C.c_int32(0)
+    rc = load().rg_mfma_plan(n, rp.ctypes.data_as(_pi64), ci.ctypes.data_as(_pi32), sched.ctypes.data_as(_pi32),
+                             kperm.ctypes.data_as(_pi32), C.byref(sreg))
+    if rc != RG_OK:
+        raise RsatbGpuError(rc, "mfma_plan: bad pattern")
+    return sched, kperm, sreg.value
 
 
 class Context:

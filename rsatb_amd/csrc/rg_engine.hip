@@ -112,6 +112,17 @@ struct Slot {
     // gather table of the MFMA SpMM: xrow[s * b + kk] = col_ind[s] * b + kk
     DevBuf xrow;
     bool has_xrow = false;
+    // The MFMA SpMM's own view of the slot (real, b >= 8): block rows in a
+    // processing order of compact regions of sreg rows (sched), and every
+    // row's blocks in the order of a distance-2 colouring of the block
+    // columns (kperm: the stored block behind every position; kcol and kxrow
+    // the column indices and gather table in that order). The values in that
+    // order (kvals64, kvals32) are copies remade after a write of the slot:
+    // they are current when their generation equals gen.
+    bool has_plan = false;
+    int32_t sreg = 0;
+    DevBuf sched, kperm, kcol, kxrow, kvals64, kvals32;
+    uint64_t kvals64_gen = 0, kvals32_gen = 0;
     // changes on every write of the slot (context-wide counter), so a pass
     // that keeps state derived from the slot can tell it is stale
     uint64_t gen = 0;
@@ -195,6 +206,11 @@ struct rg_context {
     // replayed moment loops (small operators): the executable graphs of the
     // last captured configuration, one per chunk of steps, and its key
     int graph_mode = 2; // 0 off, 1 always, 2 small operators (RG_GRAPH)
+    // MFMA SpMM plans (row schedule, colour-ordered blocks) for real b >= 8
+    // slots written from now on (RG_MFMA_PLAN=0 turns them off)
+    bool mfma_plan = true;
+    // set while rg_kpm_moments_noalloc runs: nothing may allocate
+    bool noalloc_active = false;
     std::vector<hipGraphExec_t> graphs;
     std::vector<uint64_t> graph_key;
 
@@ -268,15 +284,32 @@ bool vec_complex(const rg_context *ctx) {
 
 size_t vec_elem_bytes(VecMode vm) { return (vm.complex_vec ? 2 : 1) * (vm.fp32 ? 4 : 8); }
 
+// True when the slot's MFMA view (the scheduled rows and colour-ordered
+// blocks) holds current values for the precision.
+bool plan_current(const Slot &s, bool fp32) {
+    return s.has_plan && s.gen != 0 && (fp32 ? s.kvals32_gen : s.kvals64_gen) == s.gen;
+}
+
+// The operand a SpMM launch reads: the MFMA view when it is current (a
+// consistent BSR whose rows hold the same blocks in another order, plus the
+// row schedule), else the slot as stored.
 BsrDev dev_view(const Slot &s, bool fp32) {
     BsrDev d;
     d.nbr = s.nbr;
     d.b = s.b;
     d.nnzb = s.nnzb;
     d.row_ptr = static_cast<const int64_t *>(s.row_ptr.p);
+    d.is_complex = s.is_complex;
+    if (plan_current(s, fp32)) {
+        d.col_ind = static_cast<const int32_t *>(s.kcol.p);
+        d.vals = fp32 ? s.kvals32.p : s.kvals64.p;
+        d.xrow = static_cast<const int32_t *>(s.kxrow.p);
+        d.sched = static_cast<const int32_t *>(s.sched.p);
+        d.sreg = s.sreg;
+        return d;
+    }
     d.col_ind = static_cast<const int32_t *>(s.col_ind.p);
     d.vals = fp32 ? s.vals32.p : s.vals64.p;
-    d.is_complex = s.is_complex;
     if (s.has_xrow) d.xrow = static_cast<const int32_t *>(s.xrow.p);
     return d;
 }
@@ -291,26 +324,227 @@ double matrix_flops(const Slot &s, VecMode vm, int R) {
     return 2.0 * (double)s.nnzb * s.b * s.b * (double)R * c;
 }
 
+// ---- MFMA SpMM plan: row schedule and block order (host side)
+//
+// One launch of the MFMA SpMM keeps 128 to 256 block rows in flight per XCD
+// (one or two workgroups of 4 one-row waves per CU), and every row gathers
+// the X rows of all its block columns through that XCD's 4 MB L2. At R = 128
+// complex one atom's X rows are 26.6 KB, so the L2 holds about 150 atoms:
+// neighbours are reused from it only when the rows in flight together share
+// them and ask for them at about the same time. The plan arranges both:
+//   - rows are processed in regions of sreg rows (the rows one XCD has in
+//     flight), each grown breadth-first over the block graph, so a region is a
+//     compact ball whose neighbour union is small;
+//   - within every row the blocks are taken in the order of a distance-2
+//     colouring of the block columns (two columns stored in one row never
+//     share a colour), so at a given point of their K loops the rows of a
+//     region gather the few atoms of one colour, most of them shared.
+// A model of the L2 as an LRU of 150 atom panels with 128 lockstep rows in
+// flight, fed with the diamond 24^3 second-neighbour pattern, puts the X
+// re-fetch at about 10x for the stored order (measured: 8.8x) and at 3.1-3.8x
+// with 128-row regions (5.0x with 256-row regions; measured: 5.6x).
+
+struct MfmaPlan {
+    std::vector<int32_t> sched; // block rows in processing order
+    std::vector<int32_t> kperm; // per stored position: the block it takes
+    int32_t sreg = 0;
+};
+
+// Region size: 128 rows, what one XCD keeps in flight when every CU holds
+// one workgroup of four one-row waves (the FP64 kernels at sixteen tiles per
+// wave); kernels holding two workgroups per CU have two adjacent regions in
+// flight. Smaller for operators of fewer than 8 x 128 rows, so that every XCD
+// still gets a region.
+int32_t plan_region_rows(int64_t nbr) {
+    const int64_t per_xcd = (nbr + 7) / 8;
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(128, per_xcd));
+}
+
+// Regions of exactly sreg rows (the last one shorter), each grown breadth
+// first over the rows' block columns from a seed. The next seed is the most
+// recently queued unvisited neighbour of the previous region, taken from the
+// seed's side first, else the lowest unvisited row.
+std::vector<int32_t> region_order(int64_t n, const int64_t *rp, const int32_t *ci, int32_t sreg) {
+    std::vector<int32_t> order;
+    order.reserve((size_t)n);
+    std::vector<uint8_t> vis((size_t)n, 0);
+    std::vector<int32_t> seeds; // a stack: the back is the next candidate
+    int64_t scan = 0;
+    while ((int64_t)order.size() < n) {
+        const size_t start = order.size();
+        size_t head = start; // the breadth-first queue is order[head ..]
+        while (order.size() - start < (size_t)sreg && (int64_t)order.size() < n) {
+            if (head == order.size()) {
+                int32_t sd = -1;
+                while (!seeds.empty()) {
+                    const int32_t c = seeds.back();
+                    seeds.pop_back();
+                    if (!vis[(size_t)c]) {
+                        sd = c;
+                        break;
+                    }
+                }
+                if (sd < 0) {
+                    while (vis[(size_t)scan]) ++scan;
+                    sd = (int32_t)scan;
+                }
+                vis[(size_t)sd] = 1;
+                order.push_back(sd);
+                continue;
+            }
+            const int32_t a = order[head++];
+            for (int64_t p = rp[a]; p < rp[a + 1] && order.size() - start < (size_t)sreg; ++p) {
+                const int32_t c = ci[p];
+                if (c >= 0 && c < n && !vis[(size_t)c]) {
+                    vis[(size_t)c] = 1;
+                    order.push_back(c);
+                }
+            }
+        }
+        // candidates for the next seed: the region's unvisited neighbours,
+        // pushed so that the seed's own come out first
+        for (size_t k = order.size(); k-- > start;) {
+            const int32_t a = order[k];
+            for (int64_t p = rp[a + 1]; p-- > rp[a];) {
+                const int32_t c = ci[p];
+                if (c >= 0 && c < n && !vis[(size_t)c]) seeds.push_back(c);
+            }
+        }
+        if (seeds.size() > (size_t)4 * (size_t)n) seeds.erase(seeds.begin(), seeds.end() - (ptrdiff_t)n);
+    }
+    return order;
+}
+
+// Greedy distance-2 colouring of the block columns: columns stored in one
+// row get distinct colours. Columns are coloured in the order they first
+// appear in the schedule, each with the smallest colour no column sharing a
+// row with it holds.
+std::vector<int32_t> d2_colouring(int64_t n, const int64_t *rp, const int32_t *ci,
+                                  const std::vector<int32_t> &order) {
+    const int64_t nnz = rp[n];
+    std::vector<int64_t> tp((size_t)n + 1, 0); // rows holding column c: tr[tp[c] .. tp[c+1])
+    for (int64_t p = 0; p < nnz; ++p) ++tp[(size_t)ci[p] + 1];
+    for (int64_t i = 0; i < n; ++i) tp[(size_t)i + 1] += tp[(size_t)i];
+    std::vector<int32_t> tr((size_t)nnz);
+    {
+        std::vector<int64_t> pos(tp.begin(), tp.end() - 1);
+        for (int64_t r = 0; r < n; ++r)
+            for (int64_t p = rp[r]; p < rp[r + 1]; ++p) tr[(size_t)pos[(size_t)ci[p]]++] = (int32_t)r;
+    }
+    std::vector<int32_t> col((size_t)n, -1);
+    std::vector<int64_t> mark; // mark[k] == stamp: colour k is taken near the column
+    int64_t stamp = 0;
+    for (int32_t r : order)
+        for (int64_t p = rp[r]; p < rp[r + 1]; ++p) {
+            const int32_t c = ci[p];
+            if (col[(size_t)c] >= 0) continue;
+            ++stamp;
+            for (int64_t t = tp[(size_t)c]; t < tp[(size_t)c + 1]; ++t) {
+                const int32_t rr = tr[(size_t)t];
+                for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) {
+                    const int32_t k = col[(size_t)ci[q]];
+                    if (k < 0) continue;
+                    if ((size_t)k >= mark.size()) mark.resize((size_t)k + 1, 0);
+                    mark[(size_t)k] = stamp;
+                }
+            }
+            int32_t k = 0;
+            while ((size_t)k < mark.size() && mark[(size_t)k] == stamp) ++k;
+            col[(size_t)c] = k;
+        }
+    return col;
+}
+
+// The plan of a square pattern with columns in [0, n): the schedule, and
+// every row's blocks ordered by (colour, column, stored position).
+MfmaPlan mfma_plan(int64_t n, const int64_t *rp, const int32_t *ci, int32_t sreg) {
+    MfmaPlan pl;
+    pl.sreg = sreg;
+    pl.sched = region_order(n, rp, ci, sreg);
+    const std::vector<int32_t> col = d2_colouring(n, rp, ci, pl.sched);
+    pl.kperm.resize((size_t)rp[n]);
+    for (int64_t r = 0; r < n; ++r) {
+        int32_t *k0 = pl.kperm.data() + rp[r], *k1 = pl.kperm.data() + rp[r + 1];
+        for (int64_t p = rp[r]; p < rp[r + 1]; ++p) pl.kperm[(size_t)p] = (int32_t)p;
+        std::sort(k0, k1, [&](int32_t x, int32_t y) {
+            const int32_t cx = col[(size_t)ci[x]], cy = col[(size_t)ci[y]];
+            if (cx != cy) return cx < cy;
+            if (ci[x] != ci[y]) return ci[x] < ci[y];
+            return x < y;
+        });
+    }
+    return pl;
+}
+
 // Gather table (see BsrDev::xrow) of a real slot with b >= 8, the blocks the
-// MFMA SpMM takes; 32 trailing entries keep the steps it fetches past the last
-// row's K (at most 30 entries) in range.
-int build_xrow(rg_context *ctx, Slot &s, const int32_t *ci) {
+// MFMA SpMM takes; kXrowSpare trailing entries keep the steps it fetches past
+// the last row's K in range.
+int build_xrow(rg_context *ctx, Slot &s, const int64_t *rp, const int32_t *ci) {
     s.has_xrow = false;
+    s.has_plan = false;
     if (s.b < 8 || s.is_complex) return RG_OK;
     const int64_t n = s.nnzb * s.b;
     if ((int64_t)s.nbr * s.b > INT32_MAX) return RG_OK;
-    std::vector<int32_t> t((size_t)n + 32, 0);
+    std::vector<int32_t> t((size_t)n + kXrowSpare, 0);
     for (int64_t q = 0; q < s.nnzb; ++q)
         for (int kk = 0; kk < s.b; ++kk) t[(size_t)(q * s.b + kk)] = ci[q] * s.b + kk;
     RG_HIP(ctx, s.xrow.ensure(t.size() * 4), "hipMalloc gather table");
     RG_HIP(ctx, hipMemcpy(s.xrow.p, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload gather table");
     s.has_xrow = true;
+    if (!ctx->mfma_plan || s.nnzb >= INT32_MAX || s.nbr >= INT32_MAX) return RG_OK;
+    MfmaPlan pl = mfma_plan(s.nbr, rp, ci, plan_region_rows(s.nbr));
+    std::vector<int32_t> kc((size_t)s.nnzb + 1, 0);
+    for (int64_t q = 0; q < s.nnzb; ++q) {
+        kc[(size_t)q] = ci[pl.kperm[(size_t)q]];
+        for (int kk = 0; kk < s.b; ++kk) t[(size_t)(q * s.b + kk)] = kc[(size_t)q] * s.b + kk;
+    }
+    RG_HIP(ctx, s.sched.ensure((size_t)s.nbr * 4), "hipMalloc row schedule");
+    RG_HIP(ctx, s.kperm.ensure((size_t)s.nnzb * 4 + 4), "hipMalloc block order");
+    RG_HIP(ctx, s.kcol.ensure(kc.size() * 4), "hipMalloc block order");
+    RG_HIP(ctx, s.kxrow.ensure(t.size() * 4), "hipMalloc gather table");
+    RG_HIP(ctx, hipMemcpy(s.sched.p, pl.sched.data(), (size_t)s.nbr * 4, hipMemcpyHostToDevice), "upload plan");
+    if (s.nnzb > 0)
+        RG_HIP(ctx, hipMemcpy(s.kperm.p, pl.kperm.data(), (size_t)s.nnzb * 4, hipMemcpyHostToDevice),
+               "upload plan");
+    RG_HIP(ctx, hipMemcpy(s.kcol.p, kc.data(), kc.size() * 4, hipMemcpyHostToDevice), "upload plan");
+    RG_HIP(ctx, hipMemcpy(s.kxrow.p, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload plan");
+    s.sreg = pl.sreg;
+    s.kvals64_gen = s.kvals32_gen = 0;
+    s.has_plan = true;
+    return RG_OK;
+}
+
+// Brings the slot's MFMA view values up to date (FP64 copy, then the FP32
+// one from it). With `noalloc` only buffers that already exist are written;
+// a view that would need an allocation is left stale, and the launch reads
+// the slot as stored.
+int ensure_plan_values(rg_context *ctx, Slot &s, bool fp32, bool noalloc) {
+    if (!s.has_plan || !s.present || plan_current(s, fp32)) return RG_OK;
+    const int64_t m = (int64_t)s.b * s.b; // real slots only
+    const size_t n64 = (size_t)(s.nnzb * m) * 8, n32 = (size_t)(s.nnzb * m) * 4;
+    if (noalloc && (s.kvals64.bytes < n64 || (fp32 && s.kvals32.bytes < n32))) return RG_OK;
+    if (s.kvals64_gen != s.gen) {
+        RG_HIP(ctx, s.kvals64.ensure(n64), "hipMalloc ordered values");
+        RG_HIP(ctx, launch_block_perm(static_cast<const double *>(s.vals64.p), static_cast<const int32_t *>(s.kperm.p),
+                                      static_cast<double *>(s.kvals64.p), s.nnzb, m, ctx->stream),
+               "ordered values");
+        s.kvals64_gen = s.gen;
+    }
+    if (fp32 && s.kvals32_gen != s.gen) {
+        RG_HIP(ctx, s.kvals32.ensure(n32), "hipMalloc ordered values");
+        RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.kvals64.p), static_cast<float *>(s.kvals32.p),
+                                      s.nnzb * m, ctx->stream),
+               "ordered values");
+        s.kvals32_gen = s.gen;
+    }
     return RG_OK;
 }
 
 // One SpMM launch with optional profiling; partials go to ctx->partials.
 int spmm(rg_context *ctx, int slot, VecMode vm, const void *X, void *Y, int R, EpiArgs e,
          int *grid_out) {
+    int rcp = ensure_plan_values(ctx, ctx->slots[slot], vm.fp32, ctx->noalloc_active);
+    if (rcp) return rcp;
     const Slot &s = ctx->slots[slot];
     const int cls = e.mode == EPI_PLAIN ? RG_KCLASS_SPMM_PLAIN : RG_KCLASS_SPMM_STEP;
     hipEvent_t a = nullptr, b = nullptr;
@@ -670,6 +904,17 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
             if (rc) return rc;
         }
     }
+    // the launches read the MFMA views made current here (a captured or
+    // replayed loop must not remake them), and allocate nothing under noalloc
+    struct NoallocScope {
+        rg_context *c;
+        ~NoallocScope() { c->noalloc_active = false; }
+    } noalloc_scope{ctx};
+    ctx->noalloc_active = noalloc;
+    for (int i = 0; i < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++i) {
+        rc = ensure_plan_values(ctx, ctx->slots[i], vm.fp32, noalloc);
+        if (rc) return rc;
+    }
     const size_t nb = (size_t)rows * R * vec_elem_bytes(vm);
     const int nsteps = doubled ? std::max(1, M / 2 + 1) : std::max(1, M);
     // work vectors t0, t1, t2, the sandwich intermediates, the psi copy of the
@@ -788,10 +1033,10 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
                addr(ctx->ckpt.p), addr(wpart), addr(wraw), addr(dparts), (uint64_t)pstride};
         for (void *w : wv) key.push_back(addr(w));
         for (int sl = 0; sl < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++sl) {
-            const Slot &q = ctx->slots[sl];
-            for (uint64_t x : {addr(q.row_ptr.p), addr(q.col_ind.p), addr(vm.fp32 ? q.vals32.p : q.vals64.p),
-                               addr(q.has_xrow ? q.xrow.p : nullptr), (uint64_t)q.nbr, (uint64_t)q.b,
-                               (uint64_t)q.nnzb, (uint64_t)q.is_complex})
+            const BsrDev q = dev_view(ctx->slots[sl], vm.fp32);
+            for (uint64_t x : {addr(q.row_ptr), addr(q.col_ind), addr(q.vals), addr(q.xrow), addr(q.sched),
+                               (uint64_t)q.sreg, (uint64_t)q.nbr, (uint64_t)q.b, (uint64_t)q.nnzb,
+                               (uint64_t)q.is_complex})
                 key.push_back(x);
         }
     }
@@ -965,6 +1210,13 @@ int rg_create(int device_id, rg_context **out) {
         ctx->graph_mode = (m == 0 || m == 1) ? m : 2;
     }
     if (const char *v = std::getenv("RG_B1_WIDE")) set_b1_wide(std::atoi(v));
+    if (const char *v = std::getenv("RG_MFMA_PLAN")) ctx->mfma_plan = std::atoi(v) != 0;
+    if (const char *v = std::getenv("RG_MFMA_PD16")) {
+        // "fp64 plain, fp64 step, fp32 plain, fp32 step"
+        int p[4] = {0, 0, 0, 0};
+        if (std::sscanf(v, "%d,%d,%d,%d", &p[0], &p[1], &p[2], &p[3]) == 4)
+            for (int q = 0; q < 4; ++q) set_mfma_pd16(q / 2, q % 2, p[q]);
+    }
     if (const char *v = std::getenv("RG_MFMA_PD32")) set_mfma_pd32(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD64")) set_mfma_pd64(std::atoi(v));
     if (const char *v = std::getenv("RG_KG_WIDEN_BYTES")) {
@@ -1051,7 +1303,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     s.b = b;
     s.nnzb = nnzb;
     s.is_complex = (dtype == RG_C128);
-    int rcg = build_xrow(ctx, s, col_ind);
+    int rcg = build_xrow(ctx, s, row_ptr, col_ind);
     if (rcg) return rcg;
     s.present = true;
     return RG_OK;
@@ -1583,7 +1835,7 @@ int sk_build(rg_context *ctx, int slot, const rg_sk_table *t, int64_t n_atoms, c
         s.b = b;
         s.nnzb = nnzb;
         s.is_complex = cplx;
-        int rcg = build_xrow(ctx, s, col_ind.data());
+        int rcg = build_xrow(ctx, s, row_ptr.data(), col_ind.data());
         if (rcg) return rcg;
         s.present = true;
     }
@@ -2142,7 +2394,7 @@ int rg_isqrt_patches(rg_context *ctx, double tol, int maxit, int *iters_out, dou
         t->b = b;
         t->nnzb = S.nnzb;
         t->is_complex = false;
-        int rcg = build_xrow(ctx, *t, ci.data());
+        int rcg = build_xrow(ctx, *t, rp.data(), ci.data());
         if (rcg) return rcg;
     }
     int it_max = 0;
@@ -2731,6 +2983,23 @@ int rg_dos_plan(double emin, double emax, double de, int *n_points, int *n_momen
     const int N = std::max(1, (int)std::ceil(jackson_fwhm * half_width / de));
     if (n_points) *n_points = 2 * nt;
     if (n_moments) *n_moments = N;
+    return RG_OK;
+}
+
+int rg_mfma_plan(int64_t nbr, const int64_t *row_ptr, const int32_t *col_ind, int32_t *sched_out,
+                 int32_t *kperm_out, int32_t *sreg_out) {
+    if (nbr < 1 || nbr >= INT32_MAX || !row_ptr || !col_ind || !sched_out || !kperm_out || !sreg_out)
+        return RG_ERR_INVALID;
+    if (row_ptr[0] != 0) return RG_ERR_INVALID;
+    for (int64_t i = 0; i < nbr; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) return RG_ERR_INVALID;
+    if (row_ptr[nbr] >= INT32_MAX) return RG_ERR_UNSUPPORTED;
+    for (int64_t p = 0; p < row_ptr[nbr]; ++p)
+        if (col_ind[p] < 0 || col_ind[p] >= nbr) return RG_ERR_INVALID;
+    const MfmaPlan pl = mfma_plan(nbr, row_ptr, col_ind, plan_region_rows(nbr));
+    std::copy(pl.sched.begin(), pl.sched.end(), sched_out);
+    std::copy(pl.kperm.begin(), pl.kperm.end(), kperm_out);
+    *sreg_out = pl.sreg;
     return RG_OK;
 }
 

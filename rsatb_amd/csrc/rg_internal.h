@@ -6,6 +6,11 @@
 
 namespace rg {
 
+// Spare (zero) entries after the MFMA SpMM's gather table: a wave fetches up
+// to 2 PD - 2 steps of 4 entries past its row, and reads the entries of the
+// step after its last fetch (PD <= 6: at most 8 PD - 2 = 46 entries).
+constexpr int kXrowSpare = 64;
+
 // Device view of one BSR operand. Values are real (vals) or complex
 // interleaved (vals with is_complex), in double or float storage.
 struct BsrDev {
@@ -17,8 +22,13 @@ struct BsrDev {
     const void *vals = nullptr; // double / double2 / float / float2
     bool is_complex = false;
     // gather table of real b >= 8 matrices, or null: xrow[s * b + kk] =
-    // col_ind[s] * b + kk, the X row of column kk of block s (32 spare entries)
+    // col_ind[s] * b + kk, the X row of column kk of block s (kXrowSpare spare entries)
     const int32_t *xrow = nullptr;
+    // processing order of the block rows (MFMA SpMM), or null for stored
+    // order: position p of [0, nbr) computes block row sched[p]. The XCD
+    // ranges of a launch are whole regions of sreg consecutive positions.
+    const int32_t *sched = nullptr;
+    int32_t sreg = 0;
 };
 
 // Epilogue of a SpMM launch. y = sum_blocks A x is finished as
@@ -69,6 +79,9 @@ void set_mfma_grid_cap(int g);
 void set_b1_wide(int mode);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
+// prefetch depth (2 .. 6) of the sixteen-tile complex MFMA SpMM for one
+// precision and kind (plain product or step)
+void set_mfma_pd16(int fp32, int step, int p);
 
 // Device matrices hold every b x b block column-major; uploads arrive
 // row-major (rsatb's layout) and are transposed on the device.
@@ -96,6 +109,10 @@ hipError_t launch_reduce_steps(const double2 *partials, int64_t pstride, int ngr
 // random phase (complex). scale = 1/sqrt(rows) computed on the host.
 hipError_t launch_samples(int kind, uint64_t seed, int64_t rows, int R, double scale,
                           bool complex_vec, void *out, hipStream_t stream);
+
+// out[p * m + e] = in[perm[p] * m + e] for blocks p < nblk of m scalars.
+hipError_t launch_block_perm(const double *in, const int32_t *perm, double *out, int64_t nblk, int64_t m,
+                             hipStream_t stream);
 
 // Elementwise conversion double -> float over n scalars.
 hipError_t launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t stream);

@@ -58,6 +58,11 @@ int g_b1_wide = 2;
 // plain 54.0 and step 62.2 at 3, 70.8 at 4.
 int g_mfma_pd64 = 3;
 int g_mfma_pd32 = 3;
+// The same for sixteen tiles per wave over complex vectors, [FP64, FP32]
+// x [plain product, step]: at sixteen tiles the FP64 kernels hold one wave
+// per SIMD (128 accumulator registers), so a wave hides its gathers' latency
+// only with the steps it has in flight.
+int g_pd16[2][2] = {{3, 2}, {3, 3}};
 
 // Tiles per wave of the MFMA path for W real columns (0: not eligible).
 __host__ __device__ inline int mfma_nt(int W) {
@@ -582,7 +587,9 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
                                                    const int32_t *__restrict__ xrow,
                                                    const S *__restrict__ vals,
                                                    const S *__restrict__ X, S *__restrict__ Y,
-                                                   int W, int R, int xcd_map, EpiArgs e) {
+                                                   int W, int R, int xcd_map,
+                                                   const int32_t *__restrict__ sched, int sreg,
+                                                   EpiArgs e) {
     using A4 = typename Acc4<S>::T;
     constexpr int NDOT = MODE == EPI_FIRST ? 3 : MODE == EPI_STEP ? 2 : MODE == EPI_PLAINSTEP ? 1 : 0;
     constexpr int CW = 16 * NT; // real columns per chunk
@@ -597,14 +604,23 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     // XCD walks its own contiguous eighth of the block rows, so the rows in
     // flight on one XCD (and their gathered neighbours) stay close together
     // in its L2.
+    // With a row schedule the ranges are whole regions of sreg positions
+    // (a region is what one XCD keeps in flight), and position p computes
+    // block row sched[p].
     int64_t r_lo = 0, r_hi = nbr;
     int64_t kq = blockIdx.x, nk = gridDim.x;
     if (xcd_map) {
         const int x = blockIdx.x & 7;
         kq = blockIdx.x >> 3;
         nk = gridDim.x >> 3;
-        r_lo = nbr * x / 8;
-        r_hi = nbr * (x + 1) / 8;
+        if (sched) {
+            const int64_t nreg = (nbr + sreg - 1) / sreg;
+            r_lo = min(nbr, nreg * x / 8 * sreg);
+            r_hi = min(nbr, nreg * (x + 1) / 8 * sreg);
+        } else {
+            r_lo = nbr * x / 8;
+            r_hi = nbr * (x + 1) / 8;
+        }
     }
     const int64_t total = (r_hi - r_lo) * nchunks;
 
@@ -615,7 +631,8 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
         for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
 
     for (int64_t it = kq * kWaves + wave; it < total; it += nk * kWaves) {
-        const int64_t br = r_lo + it / nchunks;
+        const int64_t pos = r_lo + it / nchunks;
+        const int64_t br = sched ? (int64_t)sched[pos] : pos;
         const int64_t s0 = row_ptr[br];
         const int nblk = (int)(row_ptr[br + 1] - s0);
         const int nsteps = (nblk * b + 3) >> 2;
@@ -627,9 +644,9 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
         // Step st covers K columns 4 st .. 4 st + 3 of the row's column-major
         // b x K panel (element (i, k) at s0 b^2 + k b + i); lane group g takes
         // column k = 4 st + g, its X row from the gather table (wave-uniform
-        // scalar loads, so no vector load sits between the steps' loads). Two
-        // register buffers: step st + 1's operands are in flight while step
-        // st's MFMAs run. Past the row's K the table still names a valid row
+        // scalar loads, so no vector load sits between the steps' loads). PD
+        // register buffers: steps st + 1 .. st + PD - 1 are in flight while
+        // step st's MFMAs run. Past the row's K the table still names a valid row
         // and the panel read stays inside the padded value array; a = 0 there.
         // Lanes outside the panel (i >= b, or k >= K) load their A element from
         // row_ptr[0], which is 0 (checked at upload): the 8 zero bytes read as
@@ -645,7 +662,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
             const int k = 4 * fst + g;
             a = __builtin_nontemporal_load((i < b && k < K) ? panel + (int64_t)k * b + i : zero);
             load_cols<S, NT>(X + xr * W + col0, i, bv);
-            // the next step's table entries (in range: 32 spare entries)
+            // the next step's table entries (in range: the table keeps 8 PD - 2 <= 64 spare entries)
             kb += 4;
             nx0 = xrow[kb];
             nx1 = xrow[kb + 1];
@@ -680,72 +697,86 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
 }
 
-// Two workgroups per compute unit of the current device, a multiple of 8
-// (whole XCD rounds), at most kMaxGrid.
-int mfma_default_grid() {
-    int dev = 0, ncu = 0;
+// Workgroups of one MFMA SpMM kernel that are resident at once, at most two
+// per compute unit, a multiple of 8 (whole XCD rounds), at most kMaxGrid.
+// With no queued workgroups the grid-stride loop walks the schedule's regions
+// in order on every XCD.
+int mfma_resident_grid(const void *kernel) {
+    int dev = 0, ncu = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-        return kMaxGrid;
-    const int64_t g = ((int64_t)2 * ncu / 8) * 8;
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, 0) != hipSuccess || per_cu < 1)
+        return 512;
+    const int64_t g = ((int64_t)std::min(per_cu, 2) * ncu / 8) * 8;
     return (int)std::max<int64_t>(8, std::min<int64_t>(g, kMaxGrid));
 }
 
-int mfma_grid(int grid_req) {
-    static std::atomic<int> def{0};
-    int c = g_mfma_grid_cap;
-    if (c <= 0) {
-        c = def.load(std::memory_order_relaxed);
-        if (c == 0) {
-            c = mfma_default_grid();
-            def.store(c, std::memory_order_relaxed);
-        }
-    }
+// Set while spmm_launch_grid asks the MFMA dispatch which grid it would take:
+// launch_mfma then reports the grid without launching.
+thread_local bool t_grid_query = false;
+
+int mfma_grid(int grid_req, int resident) {
+    const int c = g_mfma_grid_cap > 0 ? g_mfma_grid_cap : resident;
     return grid_req < c ? grid_req : c;
+}
+
+template <typename S, int NT, int MODE, bool CV, int PD>
+hipError_t launch_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e, int grid_req,
+                       hipStream_t st, int *grid_out) {
+    static std::atomic<int> resident{0};
+    int res = resident.load(std::memory_order_relaxed);
+    if (res == 0) {
+        res = mfma_resident_grid(reinterpret_cast<const void *>(&k_spmm_mfma<S, NT, MODE, CV, PD>));
+        resident.store(res, std::memory_order_relaxed);
+    }
+    const int grid = mfma_grid(grid_req, res);
+    if (grid_out) *grid_out = grid;
+    if (t_grid_query) return hipSuccess;
+    hipLaunchKernelGGL((k_spmm_mfma<S, NT, MODE, CV, PD>), dim3(grid), dim3(64 * kWaves), 0, st, A.nbr,
+                       (int)A.b, A.row_ptr, A.xrow, static_cast<const S *>(A.vals), static_cast<const S *>(X),
+                       static_cast<S *>(Y), W, R, g_xcd_map, A.sched, A.sreg, e);
+    return hipGetLastError();
 }
 
 template <typename S, int NT, bool CV, int PD>
 hipError_t go_mfma_pd(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
-                      int grid_req, hipStream_t st, int *grid_out) {
-    const int grid = mfma_grid(grid_req);
-    if (grid_out) *grid_out = grid;
-    const int xcd = g_xcd_map;
-    const dim3 gd(grid), bd(64 * kWaves);
-    const S *vals = static_cast<const S *>(A.vals);
-    const S *x = static_cast<const S *>(X);
-    S *y = static_cast<S *>(Y);
+                      int grid, hipStream_t st, int *grid_out) {
     switch (e.mode) {
-    case EPI_PLAIN:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAIN, CV, PD>), gd, bd, 0, st, A.nbr, (int)A.b,
-                           A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
-        break;
-    case EPI_FIRST:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_FIRST, CV, PD>), gd, bd, 0, st, A.nbr, (int)A.b,
-                           A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
-        break;
-    case EPI_STEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_STEP, CV, PD>), gd, bd, 0, st, A.nbr, (int)A.b,
-                           A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
-        break;
-    case EPI_PLAINSTEP:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_PLAINSTEP, CV, PD>), gd, bd, 0, st, A.nbr,
-                           (int)A.b, A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
-        break;
-    default:
-        hipLaunchKernelGGL((k_spmm_mfma<S, NT, EPI_ADJ, CV, PD>), gd, bd, 0, st, A.nbr,
-                           (int)A.b, A.row_ptr, A.xrow, vals, x, y, W, R, xcd, e);
-        break;
+    case EPI_PLAIN: return launch_mfma<S, NT, EPI_PLAIN, CV, PD>(A, X, Y, W, R, e, grid, st, grid_out);
+    case EPI_FIRST: return launch_mfma<S, NT, EPI_FIRST, CV, PD>(A, X, Y, W, R, e, grid, st, grid_out);
+    case EPI_STEP: return launch_mfma<S, NT, EPI_STEP, CV, PD>(A, X, Y, W, R, e, grid, st, grid_out);
+    case EPI_PLAINSTEP: return launch_mfma<S, NT, EPI_PLAINSTEP, CV, PD>(A, X, Y, W, R, e, grid, st, grid_out);
+    default: return launch_mfma<S, NT, EPI_ADJ, CV, PD>(A, X, Y, W, R, e, grid, st, grid_out);
     }
-    return hipGetLastError();
 }
 
-// register prefetch depth of k_spmm_mfma (g_mfma_pd64, g_mfma_pd32)
+// Sixteen tiles per wave over complex vectors (R = 128 complex): the plain
+// product or the step at prefetch depth pd (2 .. 6).
+template <typename S, int MODE>
+hipError_t go_mfma16(int pd, const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e, int grid,
+                     hipStream_t st, int *grid_out) {
+    switch (pd) {
+    case 2: return launch_mfma<S, 16, MODE, true, 2>(A, X, Y, W, R, e, grid, st, grid_out);
+    case 4: return launch_mfma<S, 16, MODE, true, 4>(A, X, Y, W, R, e, grid, st, grid_out);
+    case 5: return launch_mfma<S, 16, MODE, true, 5>(A, X, Y, W, R, e, grid, st, grid_out);
+    case 6: return launch_mfma<S, 16, MODE, true, 6>(A, X, Y, W, R, e, grid, st, grid_out);
+    default: return launch_mfma<S, 16, MODE, true, 3>(A, X, Y, W, R, e, grid, st, grid_out);
+    }
+}
+
+// register prefetch depth of k_spmm_mfma (g_mfma_pd64, g_mfma_pd32, g_pd16)
 template <typename S, int NT, bool CV>
 hipError_t go_mfma(const BsrDev &A, const void *X, void *Y, int W, int R, const EpiArgs &e,
                    int grid, hipStream_t st, int *grid_out) {
+    if constexpr (NT == 16 && CV) {
+        const int ty = sizeof(S) == 8 ? 0 : 1;
+        if (e.mode == EPI_PLAIN)
+            return go_mfma16<S, EPI_PLAIN>(g_pd16[ty][0], A, X, Y, W, R, e, grid, st, grid_out);
+        if (e.mode == EPI_STEP)
+            return go_mfma16<S, EPI_STEP>(g_pd16[ty][1], A, X, Y, W, R, e, grid, st, grid_out);
+    }
     if constexpr (sizeof(S) == 8) {
-        if (g_mfma_pd64 == 3 && e.mode == EPI_PLAIN)
-            return go_mfma_pd<S, NT, CV, 3>(A, X, Y, W, R, e, grid, st, grid_out);
+        if (g_mfma_pd64 == 3 && e.mode == EPI_PLAIN) return go_mfma_pd<S, NT, CV, 3>(A, X, Y, W, R, e, grid, st, grid_out);
         return go_mfma_pd<S, NT, CV, 2>(A, X, Y, W, R, e, grid, st, grid_out);
     } else {
         if (g_mfma_pd32 == 4) return go_mfma_pd<S, NT, CV, 4>(A, X, Y, W, R, e, grid, st, grid_out);
@@ -851,6 +882,18 @@ __global__ __launch_bounds__(256) void k_f64_to_f32(const double *__restrict__ i
         out[t] = (float)in[t];
 }
 
+// Block gather: one block of m scalars per row of threads' grid-stride walk.
+__global__ __launch_bounds__(256) void k_block_perm(const double *__restrict__ in,
+                                                    const int32_t *__restrict__ perm,
+                                                    double *__restrict__ out, int64_t nblk, int64_t m) {
+    const int64_t n = nblk * m;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = t / m;
+        out[t] = in[(int64_t)perm[p] * m + (t - p * m)];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_stage(const double *__restrict__ in, int in_complex,
                                                void *__restrict__ out, int out_complex,
                                                int out_fp32, int64_t n) {
@@ -935,10 +978,15 @@ int spmm_launch_grid(const BsrDev &A, VecMode vm, int R) {
     while (nt > 2 && nt > g_mfma_ntcap) nt /= 2;
     const int nchunks = nt > 0 ? W / (16 * nt) : 0;
     if (!A.is_complex && A.b >= 8 && A.xrow && nt > 0 && kWaves % nchunks == 0) {
-        int64_t g = (A.nbr * nchunks + kWaves - 1) / kWaves;
-        if (g > kMaxGrid) g = kMaxGrid;
-        g = ((g + 7) / 8) * 8;
-        return mfma_grid((int)g);
+        // the grid the step launch of this operand takes (kernels differ in
+        // their resident capacity)
+        EpiArgs e;
+        e.mode = EPI_STEP;
+        int g = 0;
+        t_grid_query = true;
+        const hipError_t he = launch_spmm(A, vm, nullptr, nullptr, R, e, nullptr, &g);
+        t_grid_query = false;
+        return he == hipSuccess ? g : kMaxGrid;
     }
     return valu_grid(A.nbr, valu_lpr(A, vm.fp32, vm.complex_vec, R));
 }
@@ -984,6 +1032,9 @@ void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_mfma_grid_cap(int g) { g_mfma_grid_cap = g >= 8 && g <= kMaxGrid ? (g / 8) * 8 : 0; }
 void set_b1_wide(int mode) { g_b1_wide = mode == 0 || mode == 1 ? mode : 2; }
 void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
+void set_mfma_pd16(int fp32, int step, int p) {
+    if (p >= 2 && p <= 6) g_pd16[fp32 ? 1 : 0][step ? 1 : 0] = p;
+}
 void set_mfma_pd64(int p) { g_mfma_pd64 = p == 2 ? 2 : 3; }
 
 hipError_t launch_block_transpose(const double *in, double *out, int64_t nnzb, int b, bool cplx,
@@ -1017,6 +1068,13 @@ hipError_t launch_samples(int kind, uint64_t seed, int64_t rows, int R, double s
 
 hipError_t launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t stream) {
     hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_1d(n)), dim3(256), 0, stream, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_perm(const double *in, const int32_t *perm, double *out, int64_t nblk, int64_t m,
+                             hipStream_t stream) {
+    if (nblk <= 0 || m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_block_perm, dim3(grid_1d(nblk * m)), dim3(256), 0, stream, in, perm, out, nblk, m);
     return hipGetLastError();
 }
 
