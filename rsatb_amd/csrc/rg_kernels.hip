@@ -61,8 +61,13 @@ int g_mfma_pd32 = 3;
 // The same for sixteen tiles per wave over complex vectors, [FP64, FP32]
 // x [plain product, step]: at sixteen tiles the FP64 kernels hold one wave
 // per SIMD (128 accumulator registers), so a wave hides its gathers' latency
-// only with the steps it has in flight.
-int g_pd16[2][2] = {{3, 2}, {3, 3}};
+// only with the steps it has in flight. Measured at the diamond 24^3
+// sandwich, R = 128 complex, depth 2 / 3 / 4 / 5 / 6: FP64 plain 4727 /
+// 4522 / 4583 / 4845 / 4926 us, FP64 step (2 / 3 / 4 / 5 / 6) 5348 / 5474 /
+// 5510 / 5701 / 5752 us; FP32 (plain, step) at 2: 2329, 2699 us, at 3: 2414,
+// 2749 us, at 4: 2450, 3421 us. Deeper buffers do not pay: with the rows in
+// flight held to one region per XCD the limit is the L2, not the latency.
+int g_pd16[2][2] = {{3, 2}, {2, 2}};
 
 // Tiles per wave of the MFMA path for W real columns (0: not eligible).
 __host__ __device__ inline int mfma_nt(int W) {
@@ -410,8 +415,12 @@ template <typename S, int NT> __device__ __forceinline__ int col_of(int t, int i
     return (t / E) * 16 * E + i * E + t % E;
 }
 
-// v[t] = row[col_of(t, i)]
-template <typename S, int NT>
+// 16-byte vector of the non-temporal builtins (which take native vector types)
+typedef float f4nt __attribute__((ext_vector_type(4)));
+
+// v[t] = row[col_of(t, i)]; NT: a non-temporal load (data read once, kept out
+// of the way of the gathered rows in L2)
+template <typename S, int NT, bool NTL = false>
 __device__ __forceinline__ void load_cols(const S *__restrict__ row, int i, S (&v)[NT]) {
     constexpr int E = col_epl<S, NT>();
     constexpr int B = E * (int)sizeof(S);
@@ -419,8 +428,13 @@ __device__ __forceinline__ void load_cols(const S *__restrict__ row, int i, S (&
     for (int q = 0; q < NT / E; ++q) {
         const S *p = row + q * 16 * E + i * E;
         if constexpr (B == 16) {
-            const float4 w = *reinterpret_cast<const float4 *>(p);
-            __builtin_memcpy(&v[q * E], &w, 16);
+            if constexpr (NTL) {
+                const f4nt w = __builtin_nontemporal_load(reinterpret_cast<const f4nt *>(p));
+                __builtin_memcpy(&v[q * E], &w, 16);
+            } else {
+                const float4 w = *reinterpret_cast<const float4 *>(p);
+                __builtin_memcpy(&v[q * E], &w, 16);
+            }
         } else if constexpr (B == 8) {
             const float2 w = *reinterpret_cast<const float2 *>(p);
             __builtin_memcpy(&v[q * E], &w, 8);
@@ -431,7 +445,8 @@ __device__ __forceinline__ void load_cols(const S *__restrict__ row, int i, S (&
     }
 }
 
-// row[col_of(t, i)] = v[t]
+// row[col_of(t, i)] = v[t], non-temporal for 16-byte pieces (an output row
+// is not read again in the launch)
 template <typename S, int NT>
 __device__ __forceinline__ void store_cols(S *__restrict__ row, int i, const S (&v)[NT]) {
     constexpr int E = col_epl<S, NT>();
@@ -440,9 +455,9 @@ __device__ __forceinline__ void store_cols(S *__restrict__ row, int i, const S (
     for (int q = 0; q < NT / E; ++q) {
         S *p = row + q * 16 * E + i * E;
         if constexpr (B == 16) {
-            float4 w;
+            f4nt w;
             __builtin_memcpy(&w, &v[q * E], 16);
-            *reinterpret_cast<float4 *>(p) = w;
+            __builtin_nontemporal_store(w, reinterpret_cast<f4nt *>(p));
         } else if constexpr (B == 8) {
             float2 w;
             __builtin_memcpy(&w, &v[q * E], 8);
@@ -479,13 +494,13 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
         }
         if constexpr (MODE != EPI_PLAIN) {
             S xs[NT];
-            load_cols<S, NT>(XS + base, i, xs);
+            load_cols<S, NT, true>(XS + base, i, xs);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
             if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
                 S xp[NT];
-                load_cols<S, NT>(XP + base, i, xp);
+                load_cols<S, NT, true>(XP + base, i, xp);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
             }
@@ -496,7 +511,7 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     S qv[NT];
-                    load_cols<S, NT>(qs[q] + base, i, qv);
+                    load_cols<S, NT, true>(qs[q] + base, i, qv);
 #pragma unroll
                     for (int t = 0; t < NT; ++t)
                         y[t] = (S)__builtin_fma(cs[q], (double)qv[t], (double)y[t]);
@@ -528,7 +543,7 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
                 dot(1, y, xs, false);
             } else if constexpr (MODE == EPI_PLAINSTEP) {
                 S ps[NT];
-                load_cols<S, NT>(PSI + base, i, ps);
+                load_cols<S, NT, true>(PSI + base, i, ps);
                 dot(0, ps, y, false);
             }
         }
