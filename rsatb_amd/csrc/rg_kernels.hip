@@ -123,6 +123,15 @@ __device__ __forceinline__ float sub(float a, float b) { return a - b; }
 __device__ __forceinline__ double2 sub(double2 a, double2 b) { return double2{a.x - b.x, a.y - b.y}; }
 __device__ __forceinline__ float2 sub(float2 a, float2 b) { return float2{a.x - b.x, a.y - b.y}; }
 
+// s += x with the rounding error carried in c (TwoSum, Knuth 1969: exact
+// for any s and x; s + c is the sum to about twice the working precision).
+__device__ __forceinline__ void two_sum_add(double &s, double &c, double x) {
+    const double t = s + x;
+    const double bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+}
+
 // d += conj(a) b in FP64. Products of two floats are exact in FP64.
 __device__ __forceinline__ void dotacc(double2 &d, double a, double b) { d.x = __builtin_fma(a, b, d.x); }
 __device__ __forceinline__ void dotacc(double2 &d, float a, float b) { d.x = __builtin_fma((double)a, (double)b, d.x); }
@@ -132,6 +141,22 @@ __device__ __forceinline__ void dotacc(double2 &d, double2 a, double2 b) {
 }
 __device__ __forceinline__ void dotacc(double2 &d, float2 a, float2 b) {
     dotacc(d, double2{(double)a.x, (double)a.y}, double2{(double)b.x, (double)b.y});
+}
+
+// The same with the rounding errors of the sums carried in c: for FP32
+// values the products are exact, so this is the compensated dot product
+// (Dot2 of Ogita, Rump and Oishi 2005); FP64 values sum plainly.
+__device__ __forceinline__ void dotacc_c(double2 &d, double2 &, double a, double b) { dotacc(d, a, b); }
+__device__ __forceinline__ void dotacc_c(double2 &d, double2 &, double2 a, double2 b) { dotacc(d, a, b); }
+__device__ __forceinline__ void dotacc_c(double2 &d, double2 &c, float a, float b) {
+    two_sum_add(d.x, c.x, (double)a * (double)b);
+}
+__device__ __forceinline__ void dotacc_c(double2 &d, double2 &c, float2 a, float2 b) {
+    const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+    two_sum_add(d.x, c.x, ai * bi);
+    two_sum_add(d.x, c.x, ar * br);
+    two_sum_add(d.y, c.y, -ai * br);
+    two_sum_add(d.y, c.y, ar * bi);
 }
 
 __device__ __forceinline__ int dev_ndot(int mode) {
@@ -167,11 +192,23 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
     const int mode = e.mode;
     const int ndot = dev_ndot(mode);
 
-    double2 dacc[3][UPL];
+    // dot sums, and their compensation terms in the precision form (up to
+    // 128 columns: wider launches hold 4 columns per lane and would spill)
+    constexpr bool COMP = sizeof(S) == 4 && UPL <= 2;
+    double2 dacc[3][UPL], dcmp[COMP ? 3 : 1][UPL];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int j = 0; j < UPL; ++j) dacc[d][j] = double2{0.0, 0.0};
+        for (int j = 0; j < UPL; ++j) {
+            dacc[d][j] = double2{0.0, 0.0};
+            if constexpr (COMP) dcmp[d][j] = double2{0.0, 0.0};
+        }
+    auto dot = [&](int d, int j, VT a, VT b) {
+        if constexpr (COMP)
+            dotacc_c(dacc[d][j], dcmp[d][j], a, b);
+        else
+            dotacc(dacc[d][j], a, b);
+    };
 
     const int64_t rows_per_wg = (int64_t)kWaves * G;
     for (int64_t base = (int64_t)blockIdx.x * rows_per_wg; base < nbr;
@@ -255,17 +292,17 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
                 if (mode == EPI_FIRST) {
                     const VT xs = XS[idx];
                     y = affine(y, e.alpha, e.beta, xs);
-                    dotacc(dacc[0][j], xs, xs);
-                    dotacc(dacc[1][j], xs, y);
-                    dotacc(dacc[2][j], y, y);
+                    dot(0, j, xs, xs);
+                    dot(1, j, xs, y);
+                    dot(2, j, y, y);
                 } else if (mode == EPI_STEP) {
                     const VT xs = XS[idx];
                     y = sub(affine(y, e.alpha, e.beta, xs), XP[idx]);
-                    dotacc(dacc[0][j], y, y);
-                    dotacc(dacc[1][j], y, xs);
+                    dot(0, j, y, y);
+                    dot(1, j, y, xs);
                 } else if (mode == EPI_PLAINSTEP) {
                     y = sub(affine(y, e.alpha, e.beta, XS[idx]), XP[idx]);
-                    dotacc(dacc[0][j], PSI[idx], y);
+                    dot(0, j, PSI[idx], y);
                 } else if (mode == EPI_ADJ) {
                     y = sub(affine(y, e.alpha, e.beta, XS[idx]), XP[idx]);
                     y = affine(y, 1.0, e.cm, static_cast<const VT *>(e.qm)[idx]);
@@ -287,8 +324,17 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int j = 0; j < UPL; ++j) {
-                    dacc[d][j].x += __shfl_xor(dacc[d][j].x, off);
-                    dacc[d][j].y += __shfl_xor(dacc[d][j].y, off);
+                    const double ox = __shfl_xor(dacc[d][j].x, off), oy = __shfl_xor(dacc[d][j].y, off);
+                    if constexpr (COMP) {
+                        const double cx = __shfl_xor(dcmp[d][j].x, off), cy = __shfl_xor(dcmp[d][j].y, off);
+                        two_sum_add(dacc[d][j].x, dcmp[d][j].x, ox);
+                        two_sum_add(dacc[d][j].y, dcmp[d][j].y, oy);
+                        dcmp[d][j].x += cx;
+                        dcmp[d][j].y += cy;
+                    } else {
+                        dacc[d][j].x += ox;
+                        dacc[d][j].y += oy;
+                    }
                 }
     }
     __shared__ double2 sm[kWaves][COLS];
@@ -297,7 +343,14 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
         if (d >= ndot) break;
         if (g == 0) {
 #pragma unroll
-            for (int j = 0; j < UPL; ++j) sm[wave][q * UPL + j] = dacc[d][j];
+            for (int j = 0; j < UPL; ++j) {
+                double2 v = dacc[d][j];
+                if constexpr (COMP) {
+                    v.x += dcmp[d][j].x;
+                    v.y += dcmp[d][j].y;
+                }
+                sm[wave][q * UPL + j] = v;
+            }
         }
         __syncthreads();
         for (int u = threadIdx.x; u < COLS && u < R; u += blockDim.x) {
@@ -469,11 +522,82 @@ __device__ __forceinline__ void store_cols(S *__restrict__ row, int i, const S (
     }
 }
 
+// Per-lane FP64 dot sums of the MFMA SpMM, ND dots x NT tiles. The
+// precision form (S = float) keeps a compensation term beside every sum:
+// its terms are products of two FP32 values, exact in FP64, so summing them
+// with TwoSum is the compensated dot product Dot2 (Ogita, Rump and Oishi
+// 2005) with an exact product step. The compensation terms live in LDS (a
+// slot per lane, dot and tile; cl points at this lane's first), not in
+// registers, so the kernel keeps its occupancy. The FP64 form sums with
+// FMAs, as the reference's own dots do.
+template <typename S, int ND, int NT> struct DotAcc {
+    static constexpr bool kComp = sizeof(S) == 4;
+    double s[ND][NT];
+    double *cl = nullptr;
+    __device__ __forceinline__ double &comp(int d, int t) { return cl[(d * NT + t) * 64]; }
+    // comp(d, t) += x without reading it back (each lane owns its slots)
+    __device__ __forceinline__ void comp_add(int d, int t, double x) {
+        __hip_atomic_fetch_add(&cl[(d * NT + t) * 64], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                s[d][t] = 0.0;
+                if constexpr (kComp) {
+                    if (cl) comp(d, t) = 0.0;
+                }
+            }
+    }
+    // s[d][t] += a1 b1 + a2 b2, in that order
+    __device__ __forceinline__ void add2(int d, int t, double a1, double b1, double a2, double b2) {
+        if constexpr (kComp) {
+            double err = 0.0;
+            two_sum_add(s[d][t], err, a1 * b1);
+            two_sum_add(s[d][t], err, a2 * b2);
+            comp_add(d, t, err);
+        } else {
+            s[d][t] = __builtin_fma(a2, b2, __builtin_fma(a1, b1, s[d][t]));
+        }
+    }
+    __device__ __forceinline__ void add(int d, int t, double a, double b) {
+        if constexpr (kComp) {
+            double err = 0.0;
+            two_sum_add(s[d][t], err, a * b);
+            comp_add(d, t, err);
+        } else {
+            s[d][t] = __builtin_fma(a, b, s[d][t]);
+        }
+    }
+    // the lane groups (lanes i, i + 16, i + 32, i + 48) of every tile folded
+    // into one sum, its compensation included
+    __device__ __forceinline__ double fold4(int d, int t) {
+        double v = s[d][t];
+        if constexpr (kComp) {
+            double c = comp(d, t);
+#pragma unroll
+            for (int off = 16; off < 64; off <<= 1) {
+                const double o = __shfl_xor(v, off), oc = __shfl_xor(c, off);
+                two_sum_add(v, c, o);
+                c += oc;
+            }
+            return v + c;
+        } else {
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            return v;
+        }
+    }
+};
+
+template <typename S, int NT, int MODE>
+using MfmaDots = DotAcc<S, (epi_ndot(MODE) > 0 ? epi_ndot(MODE) : 1), NT>;
+
 template <typename S, int NT, int MODE, bool CV>
 __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[NT], int64_t br,
                                               int b, int W, int col0, int i, int g,
-                                              const EpiArgs &e,
-                                              double (&dacc)[epi_ndot(MODE) > 0 ? epi_ndot(MODE) : 1][NT],
+                                              const EpiArgs &e, MfmaDots<S, NT, MODE> &dacc,
                                               S *__restrict__ Y) {
     const S *__restrict__ XS = static_cast<const S *>(e.xs);
     const S *__restrict__ XP = static_cast<const S *>(e.xp);
@@ -524,14 +648,12 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
 #pragma unroll
                     for (int t = 0; t < NT; t += 2) {
                         const double ar = a[t], ai = a[t + 1], br_ = bb[t], bi = bb[t + 1];
-                        dacc[d][t] = __builtin_fma(ar, br_, __builtin_fma(ai, bi, dacc[d][t]));
-                        if (!self)
-                            dacc[d][t + 1] = __builtin_fma(ar, bi, __builtin_fma(-ai, br_, dacc[d][t + 1]));
+                        dacc.add2(d, t, ai, bi, ar, br_);
+                        if (!self) dacc.add2(d, t + 1, -ai, br_, ar, bi);
                     }
                 } else {
 #pragma unroll
-                    for (int t = 0; t < NT; ++t)
-                        dacc[d][t] = __builtin_fma((double)a[t], (double)bb[t], dacc[d][t]);
+                    for (int t = 0; t < NT; ++t) dacc.add(d, t, (double)a[t], (double)bb[t]);
                 }
             };
             if constexpr (MODE == EPI_FIRST) {
@@ -555,29 +677,26 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
 // column are folded, then the waves sharing a column chunk, into
 // e.partials[(blockIdx.x * NDOT + d) * R + u].
 template <typename S, int NT, int MODE, bool CV>
-__device__ __forceinline__ void mfma_dot_partials(
-    double (&dacc)[epi_ndot(MODE) > 0 ? epi_ndot(MODE) : 1][NT], int wave, int i, int g,
-    int nchunks, int R, const EpiArgs &e) {
+__device__ __forceinline__ void mfma_dot_partials(MfmaDots<S, NT, MODE> &dacc, int wave, int i, int g,
+                                                  int nchunks, int R, const EpiArgs &e) {
     constexpr int NDOT = epi_ndot(MODE);
     constexpr int CW = 16 * NT;
     if constexpr (NDOT > 0) {
-        // fold the four lane groups (rows of the 16-row tile) of every column
-#pragma unroll
-        for (int d = 0; d < NDOT; ++d)
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                dacc[d][t] += __shfl_xor(dacc[d][t], 16);
-                dacc[d][t] += __shfl_xor(dacc[d][t], 32);
-            }
         // per wave: CW real columns -> CW/2 complex (or CW real) units
         constexpr int UPC = CV ? CW / 2 : CW;
         __shared__ double2 sm[kWaves][UPC];
+        // unrolled: the sums stay in registers (a loop over d would index them)
+#pragma unroll
         for (int d = 0; d < NDOT; ++d) {
+            // fold the four lane groups (rows of the 16-row tile) of every column
+            double f[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) f[t] = dacc.fold4(d, t);
             if (g == 0) {
 #pragma unroll
                 for (int t = 0; t < NT; t += (CV ? 2 : 1)) {
                     const int uu = CV ? col_of<S, NT>(t, i) / 2 : col_of<S, NT>(t, i);
-                    sm[wave][uu] = CV ? double2{dacc[d][t], dacc[d][t + 1]} : double2{dacc[d][t], 0.0};
+                    sm[wave][uu] = CV ? double2{f[t], f[t + 1]} : double2{f[t], 0.0};
                 }
             }
             __syncthreads();
@@ -639,11 +758,12 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     }
     const int64_t total = (r_hi - r_lo) * nchunks;
 
-    double dacc[NDOT > 0 ? NDOT : 1][NT];
-#pragma unroll
-    for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) dacc[d][t] = 0.0;
+    MfmaDots<S, NT, MODE> dacc;
+    if constexpr (MfmaDots<S, NT, MODE>::kComp && NDOT > 0) {
+        __shared__ double comp_lds[kWaves * NDOT * NT * 64];
+        dacc.cl = comp_lds + wave * NDOT * NT * 64 + lane;
+    }
+    dacc.zero();
 
     for (int64_t it = kq * kWaves + wave; it < total; it += nk * kWaves) {
         const int64_t pos = r_lo + it / nchunks;

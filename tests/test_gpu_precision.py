@@ -77,3 +77,48 @@ def test_fp32_matches_fp64_on_exact_fixture(gpu_ctx):
     mu = gpu_ctx.kpm_moments(64, _lib.PREC_FP32)
     exact = np.array([np.cos(k * np.pi / 3.0) if k % 2 == 0 else 0.0 for k in range(64)])
     assert np.max(np.abs(mu - exact)) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["mfma_b13_R128", "valu_b1_R16"])
+def test_fp32_dots_are_compensated(monkeypatch, case):
+    # mu_0 of the precision form is <psi|psi> over the FP32-rounded samples:
+    # a sum of products of FP32 values, each exact in FP64. With samples over
+    # 2^-20 .. 2^20 and long per-lane sums (the MFMA case runs on 8
+    # workgroups: about 1,700 terms per lane) a plain FP64 sum drifts by many
+    # ulps; the compensated sums (TwoSum per term, per lane, then compensated
+    # folds and workgroup sums) stay within a few ulps of the correctly
+    # rounded sum.
+    import math
+
+    rng = np.random.default_rng(5)
+    if case == "mfma_b13_R128":
+        monkeypatch.setenv("RG_MFMA_GRID", "8")
+        st = diamond_supercell(12)  # 13,824 atoms x 13 orbitals, MFMA kernel with 16 tiles
+        H = assemble(st, neighbor_edges(st, 2.7), RandomBlockModel(), "H")
+        R = 128
+    else:
+        st = graphene_rect_supercell(64, 64)
+        H = assemble(st, neighbor_edges(st, 1.6), SkPzModel())
+        R = 16
+    rows = H.n_rows
+    scale = 2.0 ** rng.integers(-20, 21, size=(rows, R))
+    psi = (rng.standard_normal((rows, R)) + 1j * rng.standard_normal((rows, R))) * scale
+    ctx = _lib.Context(0)
+    try:
+        ctx.upload_bsr(_lib.SLOT_H, H)
+        ctx.set_operator(_lib.FORM_ORTHOGONAL, 0.0, 64.0)
+        ctx.set_samples(rows, R, _lib.SAMPLES_UPLOAD_COMPLEX, data=psi)
+        _, cols = ctx.kpm_moments(4, _lib.PREC_FP32, per_column=True)
+    finally:
+        ctx.close()
+        if case == "mfma_b13_R128":  # the process-wide grid cap back to its default
+            monkeypatch.setenv("RG_MFMA_GRID", "0")
+            _lib.Context(0).close()
+    f = psi.astype(np.complex64)
+    exact = np.array([math.fsum(np.concatenate([f[:, u].real.astype(np.float64) ** 2,
+                                                f[:, u].imag.astype(np.float64) ** 2])) for u in range(R)])
+    rel = np.abs(cols[0].real - exact) / exact
+    print(f"\n{case}: max rel. error of mu_0 over {rows} rows: {rel.max():.2e} "
+          f"({rel.max() / 2.0 ** -53:.1f} ulp)", flush=True)
+    assert np.all(np.abs(cols[0].imag) == 0.0)
+    assert rel.max() <= 4 * 2.0 ** -53
