@@ -143,22 +143,6 @@ __device__ __forceinline__ void dotacc(double2 &d, float2 a, float2 b) {
     dotacc(d, double2{(double)a.x, (double)a.y}, double2{(double)b.x, (double)b.y});
 }
 
-// The same with the rounding errors of the sums carried in c: for FP32
-// values the products are exact, so this is the compensated dot product
-// (Dot2 of Ogita, Rump and Oishi 2005); FP64 values sum plainly.
-__device__ __forceinline__ void dotacc_c(double2 &d, double2 &, double a, double b) { dotacc(d, a, b); }
-__device__ __forceinline__ void dotacc_c(double2 &d, double2 &, double2 a, double2 b) { dotacc(d, a, b); }
-__device__ __forceinline__ void dotacc_c(double2 &d, double2 &c, float a, float b) {
-    two_sum_add(d.x, c.x, (double)a * (double)b);
-}
-__device__ __forceinline__ void dotacc_c(double2 &d, double2 &c, float2 a, float2 b) {
-    const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
-    two_sum_add(d.x, c.x, ai * bi);
-    two_sum_add(d.x, c.x, ar * br);
-    two_sum_add(d.y, c.y, -ai * br);
-    two_sum_add(d.y, c.y, ar * bi);
-}
-
 __device__ __forceinline__ int dev_ndot(int mode) {
     return mode == EPI_FIRST ? 3 : mode == EPI_STEP ? 2 : mode == EPI_PLAINSTEP ? 1 : 0;
 }
@@ -192,23 +176,11 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
     const int mode = e.mode;
     const int ndot = dev_ndot(mode);
 
-    // dot sums, and their compensation terms in the precision form (up to
-    // 128 columns: wider launches hold 4 columns per lane and would spill)
-    constexpr bool COMP = sizeof(S) == 4 && UPL <= 2;
-    double2 dacc[3][UPL], dcmp[COMP ? 3 : 1][UPL];
+    double2 dacc[3][UPL];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int j = 0; j < UPL; ++j) {
-            dacc[d][j] = double2{0.0, 0.0};
-            if constexpr (COMP) dcmp[d][j] = double2{0.0, 0.0};
-        }
-    auto dot = [&](int d, int j, VT a, VT b) {
-        if constexpr (COMP)
-            dotacc_c(dacc[d][j], dcmp[d][j], a, b);
-        else
-            dotacc(dacc[d][j], a, b);
-    };
+        for (int j = 0; j < UPL; ++j) dacc[d][j] = double2{0.0, 0.0};
 
     const int64_t rows_per_wg = (int64_t)kWaves * G;
     for (int64_t base = (int64_t)blockIdx.x * rows_per_wg; base < nbr;
@@ -292,17 +264,17 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
                 if (mode == EPI_FIRST) {
                     const VT xs = XS[idx];
                     y = affine(y, e.alpha, e.beta, xs);
-                    dot(0, j, xs, xs);
-                    dot(1, j, xs, y);
-                    dot(2, j, y, y);
+                    dotacc(dacc[0][j], xs, xs);
+                    dotacc(dacc[1][j], xs, y);
+                    dotacc(dacc[2][j], y, y);
                 } else if (mode == EPI_STEP) {
                     const VT xs = XS[idx];
                     y = sub(affine(y, e.alpha, e.beta, xs), XP[idx]);
-                    dot(0, j, y, y);
-                    dot(1, j, y, xs);
+                    dotacc(dacc[0][j], y, y);
+                    dotacc(dacc[1][j], y, xs);
                 } else if (mode == EPI_PLAINSTEP) {
                     y = sub(affine(y, e.alpha, e.beta, XS[idx]), XP[idx]);
-                    dot(0, j, PSI[idx], y);
+                    dotacc(dacc[0][j], PSI[idx], y);
                 } else if (mode == EPI_ADJ) {
                     y = sub(affine(y, e.alpha, e.beta, XS[idx]), XP[idx]);
                     y = affine(y, 1.0, e.cm, static_cast<const VT *>(e.qm)[idx]);
@@ -324,17 +296,8 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int j = 0; j < UPL; ++j) {
-                    const double ox = __shfl_xor(dacc[d][j].x, off), oy = __shfl_xor(dacc[d][j].y, off);
-                    if constexpr (COMP) {
-                        const double cx = __shfl_xor(dcmp[d][j].x, off), cy = __shfl_xor(dcmp[d][j].y, off);
-                        two_sum_add(dacc[d][j].x, dcmp[d][j].x, ox);
-                        two_sum_add(dacc[d][j].y, dcmp[d][j].y, oy);
-                        dcmp[d][j].x += cx;
-                        dcmp[d][j].y += cy;
-                    } else {
-                        dacc[d][j].x += ox;
-                        dacc[d][j].y += oy;
-                    }
+                    dacc[d][j].x += __shfl_xor(dacc[d][j].x, off);
+                    dacc[d][j].y += __shfl_xor(dacc[d][j].y, off);
                 }
     }
     __shared__ double2 sm[kWaves][COLS];
@@ -343,14 +306,7 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
         if (d >= ndot) break;
         if (g == 0) {
 #pragma unroll
-            for (int j = 0; j < UPL; ++j) {
-                double2 v = dacc[d][j];
-                if constexpr (COMP) {
-                    v.x += dcmp[d][j].x;
-                    v.y += dcmp[d][j].y;
-                }
-                sm[wave][q * UPL + j] = v;
-            }
+            for (int j = 0; j < UPL; ++j) sm[wave][q * UPL + j] = dacc[d][j];
         }
         __syncthreads();
         for (int u = threadIdx.x; u < COLS && u < R; u += blockDim.x) {

@@ -85,9 +85,10 @@ def test_fp32_dots_are_compensated(monkeypatch, case):
     # a sum of products of FP32 values, each exact in FP64. With samples over
     # 2^-20 .. 2^20 and long per-lane sums (the MFMA case runs on 8
     # workgroups: about 1,700 terms per lane) a plain FP64 sum drifts by many
-    # ulps; the compensated sums (TwoSum per term, per lane, then compensated
-    # folds and workgroup sums) stay within a few ulps of the correctly
-    # rounded sum.
+    # ulps (16.7 measured); the MFMA kernels' compensated sums (TwoSum per
+    # term, per lane, then compensated folds and workgroup sums) stay within
+    # a few ulps of the correctly rounded sum. The VALU kernel (b = 1 here)
+    # sums a few rows per lane and compensates across workgroups only.
     import math
 
     rng = np.random.default_rng(5)
