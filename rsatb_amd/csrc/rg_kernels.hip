@@ -550,11 +550,15 @@ template <typename S, int ND, int NT> struct DotAcc {
 template <typename S, int NT, int MODE>
 using MfmaDots = DotAcc<S, (epi_ndot(MODE) > 0 ? epi_ndot(MODE) : 1), NT>;
 
-template <typename S, int NT, int MODE, bool CV>
+// With PRE (the FP64 step), pxs and pxp arrive holding the Xs and Xp values
+// of the first output rows (loaded by the kernel before the row's K loop), and
+// each pass loads the next pass's values into them as soon as its own are
+// consumed, so the epilogue's reads overlap its compute instead of stalling.
+template <typename S, int NT, int MODE, bool CV, bool PRE = false>
 __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[NT], int64_t br,
                                               int b, int W, int col0, int i, int g,
                                               const EpiArgs &e, MfmaDots<S, NT, MODE> &dacc,
-                                              S *__restrict__ Y) {
+                                              S *__restrict__ Y, S (&pxs)[NT], S (&pxp)[NT]) {
     const S *__restrict__ XS = static_cast<const S *>(e.xs);
     const S *__restrict__ XP = static_cast<const S *>(e.xp);
     const S *__restrict__ PSI = static_cast<const S *>(e.psi);
@@ -572,17 +576,32 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
             // element v of acc[t], selected without dynamic indexing
             y[t] = v == 0 ? acc[t][0] : v == 1 ? acc[t][1] : v == 2 ? acc[t][2] : acc[t][3];
         }
+        // the next pass's output row of this lane (PRE loads it early)
+        const int rn = sizeof(S) == 8 ? 4 * (v + 1) + g : 4 * g + v + 1;
+        const bool next = PRE && v < 3 && rn < b;
+        const int64_t base_n = (br * b + rn) * (int64_t)W + col0;
         if constexpr (MODE != EPI_PLAIN) {
             S xs[NT];
-            load_cols<S, NT, true>(XS + base, i, xs);
+            if constexpr (PRE) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) xs[t] = pxs[t];
+            } else {
+                load_cols<S, NT, true>(XS + base, i, xs);
+            }
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 y[t] = (S)__builtin_fma(e.alpha, (double)y[t], e.beta * (double)xs[t]);
             if constexpr (MODE == EPI_STEP || MODE == EPI_PLAINSTEP || MODE == EPI_ADJ) {
-                S xp[NT];
-                load_cols<S, NT, true>(XP + base, i, xp);
+                if constexpr (PRE) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
+                    for (int t = 0; t < NT; ++t) y[t] = y[t] - pxp[t];
+                    if (next) load_cols<S, NT, true>(XP + base_n, i, pxp);
+                } else {
+                    S xp[NT];
+                    load_cols<S, NT, true>(XP + base, i, xp);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) y[t] = y[t] - xp[t];
+                }
             }
             if constexpr (MODE == EPI_ADJ) {
                 const S *qs[3] = {static_cast<const S *>(e.qm), static_cast<const S *>(e.qz),
@@ -619,6 +638,9 @@ __device__ __forceinline__ void mfma_epilogue(const typename Acc4<S>::T (&acc)[N
             } else if constexpr (MODE == EPI_STEP) {
                 dot(0, y, y, true);
                 dot(1, y, xs, false);
+                if constexpr (PRE) {
+                    if (next) load_cols<S, NT, true>(XS + base_n, i, pxs);
+                }
             } else if constexpr (MODE == EPI_PLAINSTEP) {
                 S ps[NT];
                 load_cols<S, NT, true>(PSI + base, i, ps);
@@ -683,6 +705,9 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
     using A4 = typename Acc4<S>::T;
     constexpr int NDOT = MODE == EPI_FIRST ? 3 : MODE == EPI_STEP ? 2 : MODE == EPI_PLAINSTEP ? 1 : 0;
     constexpr int CW = 16 * NT; // real columns per chunk
+    // the FP64 step at sixteen tiles (one wave per SIMD, registers to spare)
+    // loads its epilogue's inputs ahead
+    constexpr bool PRE = sizeof(S) == 8 && NT == 16 && MODE == EPI_STEP;
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -727,6 +752,13 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
         const int64_t s0 = row_ptr[br];
         const int nblk = (int)(row_ptr[br + 1] - s0);
         const int nsteps = (nblk * b + 3) >> 2;
+        // the first output rows' Xs, Xp values, in flight during the K loop
+        S pxs[NT], pxp[NT];
+        if constexpr (PRE) {
+            const int64_t base0 = (br * b + g) * (int64_t)W + col0; // FP64 pass 0: row g < 8 <= b
+            load_cols<S, NT, true>(static_cast<const S *>(e.xs) + base0, i, pxs);
+            load_cols<S, NT, true>(static_cast<const S *>(e.xp) + base0, i, pxp);
+        }
 
         A4 acc[NT];
 #pragma unroll
@@ -782,7 +814,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(int64_t nbr, int b,
             }
         }
 
-        mfma_epilogue<S, NT, MODE, CV>(acc, br, b, W, col0, i, g, e, dacc, Y);
+        mfma_epilogue<S, NT, MODE, CV, PRE>(acc, br, b, W, col0, i, g, e, dacc, Y, pxs, pxp);
     }
 
     mfma_dot_partials<S, NT, MODE, CV>(dacc, wave, i, g, nchunks, R, e);
