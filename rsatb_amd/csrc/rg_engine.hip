@@ -854,6 +854,10 @@ struct Recorder {
         hipGraph_t g = nullptr;
         capturing = false;
         RG_HIP(ctx, hipStreamEndCapture(ctx->stream, &g), "hipStreamEndCapture");
+        if (!g) {
+            drop_graphs(ctx);
+            return fail(ctx, RG_ERR_HIP, "hipStreamEndCapture returned no graph");
+        }
         hipGraphExec_t ex = nullptr;
         const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);

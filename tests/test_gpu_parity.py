@@ -249,3 +249,27 @@ def test_cancel_between_launches(oracle):
         assert abs(mu[0] - 1.0) < 1e-14
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["real", "realmat_cvec"])
+@pytest.mark.parametrize("R", [1, 16, 33, 128])
+def test_b1_empty_and_long_rows(gpu_ctx, oracle, kind, R):
+    # scalar entries (the b = 1 kernel, real matrix, real or complex vectors):
+    # empty rows, rows of every length up to dense rows of 300 entries (past
+    # the 4-entry gather groups), every lane layout of R
+    rng = np.random.default_rng(R + (7 if kind == "real" else 11))
+    nbr = 300
+    lens = rng.integers(0, 9, size=nbr)
+    lens[rng.choice(nbr, 30, replace=False)] = 0
+    lens[200:210] = nbr  # dense rows
+    rows = np.repeat(np.arange(nbr), lens)
+    cols = np.concatenate([rng.choice(nbr, k, replace=False) if k < nbr else np.arange(nbr) for k in lens])
+    A = BSR.from_blocks(nbr, 1, rows, cols, rng.standard_normal((rows.size, 1, 1)))
+    X = rng.standard_normal((nbr, R))
+    if kind != "real":
+        X = X + 1j * rng.standard_normal((nbr, R))
+    gpu_ctx.upload_bsr(_lib.SLOT_H, A)
+    y = gpu_ctx.bsr_mult(_lib.SLOT_H, X)
+    ref = oracle.bsr_mult(A, X)
+    assert rel_err(y, ref) < FP64_TOL
+    assert np.array_equal(y[lens == 0], np.zeros_like(y[lens == 0]))
