@@ -57,6 +57,8 @@ def parse():
                    help="moments in each timed CPU sample (2 sandwich applies at the headline size)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 steps")
+    p.add_argument("--fp32-only", action="store_true",
+                   help="time the steps in the FP32 form only (profiling runs; the line's dtype says f32)")
     p.add_argument("--no-legs", action="store_true", help="only the headline")
     p.add_argument("--leg", default=None,
                    choices=["c2", "c1", "adjoint", "sk_assembly", "kubo_greenwood", "graphene_scale", "tblg_transport"],
@@ -657,7 +659,9 @@ def main() -> int:
     R, Ms = args.nsamples, args.step_moments
     log(f"setup {setup_s:.1f}s: {rows} orbitals, bounds {calc.energy_bounds()}, "
         f"patches {calc._kstates[0].get('isqrt')}")
-    prec64 = _lib.PREC_FP64
+    prec64 = _lib.PREC_FP32 if args.fp32_only else _lib.PREC_FP64
+    if args.fp32_only:
+        args.no_fp32 = True
 
     for _ in range(args.warmup):
         ctx.kpm_moments(Ms, prec64)
@@ -689,15 +693,16 @@ def main() -> int:
     line = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "ms_per_step_event_timed": dt_prof / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.fp32_only else "f64", "data": "synthetic",
         "config": {"workload": f"diamond {args.cells}x{args.cells}x{args.cells} conventional cells "
                                f"({st.n_atoms} atoms x 13 orbitals = {rows} orbitals), sandwich Z^H H Z with "
                                f"graph-local Z = S^-1/2, R={R} random-phase complex vectors, M={args.moments} "
                                f"moments; one step = the first {Ms} moments ({Ms // 2} doubled sandwich applies) "
                                f"at full N and R",
                    "n_orbitals": rows, "R": R, "M": args.moments, "step_moments": Ms, "form": "sandwich",
-                   "precision": "fp64", "bounds": list(calc.energy_bounds()), "setup_s": round(setup_s, 1)},
-        "roofline": roofline(stats, "fp64", tag),
+                   "precision": "fp32" if args.fp32_only else "fp64", "bounds": list(calc.energy_bounds()),
+                   "setup_s": round(setup_s, 1)},
+        "roofline": roofline(stats, "fp32" if args.fp32_only else "fp64", tag),
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "count": v["count"]} for k, v in stats.items()},
         "mu0": float(mu_step[0].real),
     }
