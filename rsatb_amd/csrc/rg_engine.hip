@@ -692,10 +692,15 @@ int check_operator(rg_context *ctx) {
 }
 
 // FP32 copy of one slot's values (kept until the slot is written again).
-int ensure_fp32_slot(rg_context *ctx, int slot) {
+// With noalloc it is remade only into a buffer that already holds it.
+int ensure_fp32_slot(rg_context *ctx, int slot, bool noalloc = false) {
     Slot &s = ctx->slots[slot];
     if (s.vals32_valid) return RG_OK;
     const int64_t n = s.nnzb * s.b * s.b * (s.is_complex ? 2 : 1);
+    if (noalloc && s.vals32.bytes < (size_t)n * 4)
+        return fail(ctx, RG_ERR_STATE,
+                    "the FP32 copies of the matrices are made by an FP32 rg_kpm_moments call "
+                    "(which allocates); make one before the no-allocation call");
     RG_HIP(ctx, s.vals32.ensure((size_t)n * 4), "hipMalloc fp32 values");
     RG_HIP(ctx, launch_f64_to_f32(static_cast<const double *>(s.vals64.p), static_cast<float *>(s.vals32.p), n,
                                   ctx->stream),
@@ -898,11 +903,10 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     vm.fp32 = (precision == RG_PREC_FP32);
     if (vm.fp32) {
         if (noalloc) {
-            for (int i = 0; i < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++i)
-                if (!ctx->slots[i].vals32_valid)
-                    return fail(ctx, RG_ERR_STATE,
-                                "the FP32 copies of the matrices are made by an FP32 rg_kpm_moments call "
-                                "(which allocates); make one before the no-allocation call");
+            for (int i = 0; i < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++i) {
+                rc = ensure_fp32_slot(ctx, i, true);
+                if (rc) return rc;
+            }
         } else {
             rc = ensure_fp32(ctx);
             if (rc) return rc;

@@ -63,6 +63,17 @@ def test_noalloc_sandwich_and_fp32():
             assert np.array_equal(ref, got) and np.array_equal(rc_, gc_)
             ref, rc_, got, gc_ = both(ctx, M, _lib.PREC_FP32)  # FP32 copies made by the first call
             assert np.array_equal(ref, got) and np.array_equal(rc_, gc_)
+        # new values into the same slot: the no-allocation call runs first and
+        # remakes the value copies made earlier (FP32 values, the MFMA products'
+        # ordered values) in their buffers; the allocating call after it reads
+        # the same copies
+        H2 = assemble(st, e, RandomBlockModel(seed=3), "H")
+        ctx.upload_bsr(_lib.SLOT_H, H2)
+        for prec in (_lib.PREC_FP64, _lib.PREC_FP32):
+            got, gc_ = ctx.kpm_moments_noalloc(40, prec, per_column=True)
+            ref, rc_ = ctx.kpm_moments(40, prec, per_column=True)
+            print(f"\nre-uploaded H, precision {prec}: max |diff| {np.max(np.abs(got - ref)):.1e}", flush=True)
+            assert np.array_equal(ref, got) and np.array_equal(rc_, gc_)
     finally:
         ctx.close()
 
