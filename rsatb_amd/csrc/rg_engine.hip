@@ -123,6 +123,8 @@ struct Slot {
     int32_t sreg = 0;
     DevBuf sched, kperm, kcol, kxrow, kvals64, kvals32;
     uint64_t kvals64_gen = 0, kvals32_gen = 0;
+    // blocks in every block row when they are all equal (0: ragged)
+    int32_t ulen = 0;
     // changes on every write of the slot (context-wide counter), so a pass
     // that keeps state derived from the slot can tell it is stale
     uint64_t gen = 0;
@@ -300,6 +302,7 @@ BsrDev dev_view(const Slot &s, bool fp32) {
     d.nnzb = s.nnzb;
     d.row_ptr = static_cast<const int64_t *>(s.row_ptr.p);
     d.is_complex = s.is_complex;
+    d.ulen = s.ulen;
     if (plan_current(s, fp32)) {
         d.col_ind = static_cast<const int32_t *>(s.kcol.p);
         d.vals = fp32 ? s.kvals32.p : s.kvals64.p;
@@ -482,6 +485,12 @@ MfmaPlan mfma_plan(int64_t n, const int64_t *rp, const int32_t *ci, int32_t sreg
 int build_xrow(rg_context *ctx, Slot &s, const int64_t *rp, const int32_t *ci) {
     s.has_xrow = false;
     s.has_plan = false;
+    s.ulen = 0;
+    if (s.nbr > 0 && rp[0] == 0 && rp[1] > 0 && rp[1] <= 64) {
+        int64_t r = 1;
+        while (r < s.nbr && rp[r + 1] - rp[r] == rp[1]) ++r;
+        if (r == s.nbr) s.ulen = (int32_t)rp[1];
+    }
     if (s.b < 8 || s.is_complex) return RG_OK;
     const int64_t n = s.nnzb * s.b;
     if ((int64_t)s.nbr * s.b > INT32_MAX) return RG_OK;
@@ -1043,7 +1052,7 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
         for (int sl = 0; sl < (ctx->form == RG_FORM_SANDWICH ? 3 : 1); ++sl) {
             const BsrDev q = dev_view(ctx->slots[sl], vm.fp32);
             for (uint64_t x : {addr(q.row_ptr), addr(q.col_ind), addr(q.vals), addr(q.xrow), addr(q.sched),
-                               (uint64_t)q.sreg, (uint64_t)q.nbr, (uint64_t)q.b, (uint64_t)q.nnzb,
+                               (uint64_t)q.sreg, (uint64_t)q.ulen, (uint64_t)q.nbr, (uint64_t)q.b, (uint64_t)q.nnzb,
                                (uint64_t)q.is_complex})
                 key.push_back(x);
         }
@@ -1218,6 +1227,7 @@ int rg_create(int device_id, rg_context **out) {
         ctx->graph_mode = (m == 0 || m == 1) ? m : 2;
     }
     if (const char *v = std::getenv("RG_B1_WIDE")) set_b1_wide(std::atoi(v));
+    if (const char *v = std::getenv("RG_ULEN")) set_uniform_rows(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PLAN")) ctx->mfma_plan = std::atoi(v) != 0;
     if (const char *v = std::getenv("RG_MFMA_PD16")) {
         // "fp64 plain, fp64 step, fp32 plain, fp32 step"

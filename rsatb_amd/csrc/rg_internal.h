@@ -29,6 +29,10 @@ struct BsrDev {
     // ranges of a launch are whole regions of sreg consecutive positions.
     const int32_t *sched = nullptr;
     int32_t sreg = 0;
+    // > 0 when every block row holds exactly ulen blocks and row_ptr[0] = 0:
+    // row br's blocks are then br * ulen .. br * ulen + ulen - 1, and the VALU
+    // SpMM computes them instead of loading row_ptr
+    int32_t ulen = 0;
 };
 
 // Epilogue of a SpMM launch. y = sum_blocks A x is finished as
@@ -77,6 +81,7 @@ void set_mfma_ntcap(int nt);
 // anything else: two workgroups per compute unit (the default)
 void set_mfma_grid_cap(int g);
 void set_b1_wide(int mode);
+void set_uniform_rows(int on);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
 // prefetch depth (2 .. 6) of the sixteen-tile complex MFMA SpMM for one

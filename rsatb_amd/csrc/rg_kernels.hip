@@ -51,6 +51,8 @@ int g_mfma_grid_cap = 0;
 // b = 1 SpMM over real vectors with 2 columns per lane: 1 always, 0 never,
 // 2 by precision and row length.
 int g_b1_wide = 2;
+// The VALU SpMM computes the entry range of uniform rows (1) or loads it (0).
+int g_uniform_rows = 1;
 // Register prefetch depths of the MFMA SpMM: FP64 plain products 3 (2 when
 // set to 2; steps always 2: their epilogue needs the registers), FP32 3 or 4.
 // Measured at the diamond 8^3 sandwich, R = 64 complex: FP64 plain 105.6 us
@@ -149,7 +151,7 @@ __device__ __forceinline__ int dev_ndot(int mode) {
 
 // ------------------------------------------------------------------ SpMM
 template <typename S, bool CM, bool CV, int BS, int LPR, int UPL>
-__global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
+__global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt, int ulen,
                                               const int64_t *__restrict__ row_ptr,
                                               const int32_t *__restrict__ col_ind,
                                               const void *__restrict__ vals_v,
@@ -194,8 +196,10 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t nbr, int bs_rt,
 #pragma unroll
             for (int j = 0; j < UPL; ++j) acc[r][j] = vzero<VT>();
 
-        const int64_t s0 = row_ptr[br];
-        const int64_t s1 = row_ptr[br + 1];
+        // uniform rows: the range follows from br, and the column index
+        // loads wait on nothing (one dependent load fewer per row)
+        const int64_t s0 = ulen > 0 ? br * ulen : row_ptr[br];
+        const int64_t s1 = ulen > 0 ? s0 + ulen : row_ptr[br + 1];
         int64_t s = s0;
         if constexpr (BS == 1) {
             // Scalar entries: the gather of each entry's X row waits on its
@@ -344,7 +348,7 @@ template <typename S, bool CM, bool CV, int BS, int LPR, int UPL>
 hipError_t go(const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, int grid,
               hipStream_t st) {
     hipLaunchKernelGGL((k_spmm<S, CM, CV, BS, LPR, UPL>), dim3(grid), dim3(64 * kWaves), 0, st,
-                       A.nbr, (int)A.b, A.row_ptr, A.col_ind, A.vals, X, Y, R, e);
+                       A.nbr, (int)A.b, g_uniform_rows ? A.ulen : 0, A.row_ptr, A.col_ind, A.vals, X, Y, R, e);
     return hipGetLastError();
 }
 
@@ -1154,6 +1158,7 @@ void set_xcd_map(int on) { g_xcd_map = on ? 1 : 0; }
 void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_mfma_grid_cap(int g) { g_mfma_grid_cap = g >= 8 && g <= kMaxGrid ? (g / 8) * 8 : 0; }
 void set_b1_wide(int mode) { g_b1_wide = mode == 0 || mode == 1 ? mode : 2; }
+void set_uniform_rows(int on) { g_uniform_rows = on != 0; }
 void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
 void set_mfma_pd16(int fp32, int step, int p) {
     if (p >= 2 && p <= 6) g_pd16[fp32 ? 1 : 0][step ? 1 : 0] = p;

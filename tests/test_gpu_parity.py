@@ -273,3 +273,42 @@ def test_b1_empty_and_long_rows(gpu_ctx, oracle, kind, R):
     ref = oracle.bsr_mult(A, X)
     assert rel_err(y, ref) < FP64_TOL
     assert np.array_equal(y[lens == 0], np.zeros_like(y[lens == 0]))
+
+
+@pytest.mark.parametrize("b,L", [(1, 1), (1, 3), (1, 4), (1, 9), (2, 3)])
+@pytest.mark.parametrize("R", [16, 33])
+def test_uniform_rows(monkeypatch, oracle, b, L, R):
+    # every block row with L blocks: the VALU SpMM computes the entry range
+    # from the row index instead of loading it, which changes no sum (bit
+    # identical to the loaded range); a ragged pattern uploaded into the same
+    # buffers afterwards goes back to the loaded range
+    rng = np.random.default_rng(100 * b + 10 * L + R)
+    nbr = 257
+    rows = np.repeat(np.arange(nbr), L)
+    cols = np.concatenate([rng.choice(nbr, L, replace=False) for _ in range(nbr)])
+    A = BSR.from_blocks(nbr, b, rows, cols, rng.standard_normal((rows.size, b, b)))
+    X = rng.standard_normal((nbr * b, R)) + 1j * rng.standard_normal((nbr * b, R))
+    lens = rng.multinomial(nbr * L, np.ones(nbr) / nbr)  # the same count, ragged (empty rows too)
+    rows2 = np.repeat(np.arange(nbr), lens)
+    cols2 = np.concatenate([rng.choice(nbr, k, replace=False) for k in lens])
+    A2 = BSR.from_blocks(nbr, b, rows2, cols2, rng.standard_normal((rows2.size, b, b)))
+    assert (lens != L).any()
+    got = {}
+    try:
+        for on in ("1", "0"):
+            monkeypatch.setenv("RG_ULEN", on)
+            ctx = _lib.Context(0)  # the knob is read at context creation (process-wide)
+            try:
+                ctx.upload_bsr(_lib.SLOT_H, A)
+                y = ctx.bsr_mult(_lib.SLOT_H, X)
+                ctx.upload_bsr(_lib.SLOT_H, A2)
+                y2 = ctx.bsr_mult(_lib.SLOT_H, X)
+            finally:
+                ctx.close()
+            got[on] = (y, y2)
+    finally:
+        monkeypatch.setenv("RG_ULEN", "1")
+        _lib.Context(0).close()
+    assert rel_err(got["1"][0], oracle.bsr_mult(A, X)) < FP64_TOL
+    assert rel_err(got["1"][1], oracle.bsr_mult(A2, X)) < FP64_TOL
+    assert np.array_equal(got["1"][0], got["0"][0]) and np.array_equal(got["1"][1], got["0"][1])
