@@ -502,7 +502,9 @@ def graphene_scale_leg(cells: int, R: int = 16, M: int = 1024) -> dict:
         c.set_samples(N, R, _lib.SAMPLES_RADEMACHER, 20240917)
         mus = {}
         for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
-            c.kpm_moments(16, prec, _lib.DOUBLING_ON)  # warm-up
+            # warm-up at full length: the first run of a length allocates its
+            # workspace (the deferred dot partials: 513 MiB here, about 6 ms)
+            c.kpm_moments(M, prec, _lib.DOUBLING_ON)
             t0 = time.perf_counter()
             mus[name] = c.kpm_moments(M, prec, _lib.DOUBLING_ON)
             dt = time.perf_counter() - t0

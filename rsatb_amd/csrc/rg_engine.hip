@@ -979,16 +979,18 @@ int kpm_run(rg_context *ctx, int M, int precision, int doubling, double *mu_out,
     double2 *raw = static_cast<double2 *>(wraw);
     const double2 *parts = static_cast<const double2 *>(wpart);
 
-    // Deferred reductions (small operators): a step's kernels take a few
-    // microseconds, of which a separate reduction launch per step is half.
-    // Each doubled step then writes its dot partials to a region of its own,
-    // and one batched launch after the loop reduces them all, in the same
-    // fixed order (the moments are the same bit for bit).
+    // Deferred reductions: a small operator's step kernels take a few
+    // microseconds, of which a separate reduction launch per step is half,
+    // and a b = 1 step over a million rows (90 us) still pays about 8 us for
+    // it. Each doubled step then writes its dot partials to a region of its
+    // own, and one batched launch after the loop reduces them all, in the
+    // same fixed order (the moments are the same bit for bit), whenever the
+    // regions of the whole run fit 1 GiB.
     const int chain_slot = ctx->form == RG_FORM_SANDWICH ? RG_SLOT_ZH : RG_SLOT_H;
     const int g_step = spmm_launch_grid(dev_view(ctx->slots[chain_slot], vm.fp32), vm, R);
     const int64_t pstride = (int64_t)g_step * 2 * R; // double2 per step region
-    const bool defer = doubled && !gd && !noalloc && nb <= ((size_t)32 << 20) && nsteps > 2 &&
-                       (size_t)nsteps * (size_t)pstride * 16 <= ((size_t)512 << 20);
+    const bool defer = doubled && !gd && !noalloc && nsteps > 2 &&
+                       (size_t)nsteps * (size_t)pstride * 16 <= ((size_t)1 << 30);
     double2 *dparts = nullptr;
     if (defer) {
         RG_HIP(ctx, ctx->dpart.ensure((size_t)nsteps * (size_t)pstride * 16), "hipMalloc step partials");
@@ -1228,6 +1230,8 @@ int rg_create(int device_id, rg_context **out) {
     }
     if (const char *v = std::getenv("RG_B1_WIDE")) set_b1_wide(std::atoi(v));
     if (const char *v = std::getenv("RG_ULEN")) set_uniform_rows(std::atoi(v));
+    if (const char *v = std::getenv("RG_B1V")) set_b1_slab(std::atoi(v));
+    if (const char *v = std::getenv("RG_B1V_MIN")) set_b1_slab_min_grid(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PLAN")) ctx->mfma_plan = std::atoi(v) != 0;
     if (const char *v = std::getenv("RG_MFMA_PD16")) {
         // "fp64 plain, fp64 step, fp32 plain, fp32 step"

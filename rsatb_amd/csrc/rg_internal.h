@@ -82,6 +82,12 @@ void set_mfma_ntcap(int nt);
 void set_mfma_grid_cap(int g);
 void set_b1_wide(int mode);
 void set_uniform_rows(int on);
+// bytes of a row one lane owns in the b = 1 slab SpMM: 16, 32, 64, 0 (off);
+// any other value restores the default
+void set_b1_slab(int bytes);
+// fewest workgroups of a slab launch (smaller problems take the other
+// layouts); negative: the default
+void set_b1_slab_min_grid(int g);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
 // prefetch depth (2 .. 6) of the sixteen-tile complex MFMA SpMM for one

@@ -53,6 +53,17 @@ int g_mfma_grid_cap = 0;
 int g_b1_wide = 2;
 // The VALU SpMM computes the entry range of uniform rows (1) or loads it (0).
 int g_uniform_rows = 1;
+// Bytes of a row one lane owns in the b = 1 slab SpMM (16, 32 or 64; 0: the
+// column-per-lane layout of k_spmm), and the fewest workgroups a slab launch
+// may have (below it the rows go to k_spmm, whose layouts spread them over
+// more waves). Measured on graphene p_z, R = 16 real, FP64 / FP32 step at
+// 1,048,576 rows: k_spmm 134.1 / 69.7 us, slabs of 16 bytes 92.3 / 42.3 us
+// (0.63 / 0.72 of the HBM peak), 32 bytes 99.2 / 45.0, 64 bytes 117.7 / 65.8;
+// the 16,384-row graphene run (FP64 / FP32) 2.90 / 2.73 ms with k_spmm,
+// 2.61 / 2.86 ms with 16-byte slabs: the FP32 slab launch there has 256
+// workgroups, one wave per SIMD.
+int g_b1v_bytes = 16;
+int g_b1v_min_grid = 512;
 // Register prefetch depths of the MFMA SpMM: FP64 plain products 3 (2 when
 // set to 2; steps always 2: their epilogue needs the registers), FP32 3 or 4.
 // Measured at the diamond 8^3 sandwich, R = 64 complex: FP64 plain 105.6 us
@@ -339,7 +350,9 @@ bool b1_wide(const BsrDev &A, bool fp32, bool cv, int R) {
 }
 
 // Lanes per block row of the VALU SpMM launch for this operand.
+int b1v_lpr(const BsrDev &A, bool fp32, bool cv, int R);
 int valu_lpr(const BsrDev &A, bool fp32, bool cv, int R) {
+    if (const int l = b1v_lpr(A, fp32, cv, R)) return l;
     if (b1_wide(A, fp32, cv, R)) return R <= 16 ? 8 : 16;
     return lpr_for(R);
 }
@@ -385,6 +398,237 @@ hipError_t by_block(const BsrDev &A, const void *X, void *Y, int R, const EpiArg
     if (A.b == 1) return by_layout<S, CM, CV, 1>(A, X, Y, R, e, grid, st);
     if (A.b == 13) return by_layout<S, CM, CV, 13>(A, X, Y, R, e, grid, st);
     return by_layout<S, CM, CV, 0>(A, X, Y, R, e, grid, st);
+}
+
+// ------------------------------------------------------- b = 1 slab SpMM
+// Scalar entries, real matrix, R = LPR * CPL exactly: lane q of a row's LPR
+// lanes owns the contiguous slab of columns q * CPL .. q * CPL + CPL - 1 and
+// moves it as 16-byte loads and stores (CPL values of 8 or 16 bytes: 16, 32
+// or 64 bytes a lane). Fewer lanes per row put more rows, and more
+// independent gathers, in one wave than the column-per-lane layout of
+// k_spmm; every product and epilogue value is formed by the same operations
+// in the same order as there (the dots differ only in their summation
+// order). Vector operands must be 16-byte aligned.
+template <typename S, bool CV, int LPR, int CPL>
+__global__ __launch_bounds__(256) void k_spmm_b1v(int64_t nbr, int ulen, const int64_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col_ind,
+                                                  const S *__restrict__ vals, const S *__restrict__ X,
+                                                  S *__restrict__ Y, EpiArgs e) {
+    using VT = std::conditional_t<CV, typename V2<S>::T, S>;
+    constexpr int G = 64 / LPR;
+    constexpr int NS = CPL * (CV ? 2 : 1);    // scalars a lane owns in a row
+    constexpr int EPC = 16 / (int)sizeof(S);  // scalars a 16-byte chunk holds
+    constexpr int NCH = NS / EPC;             // chunks a lane moves per row
+    static_assert(NS % EPC == 0 && 64 % LPR == 0, "slab of whole 16-byte chunks");
+    typedef S C16 __attribute__((ext_vector_type(EPC)));
+    constexpr int64_t W = (int64_t)LPR * NS;  // scalars a row holds
+
+    const S *__restrict__ XS = static_cast<const S *>(e.xs);
+    const S *__restrict__ XP = static_cast<const S *>(e.xp);
+    const S *__restrict__ PSI = static_cast<const S *>(e.psi);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = lane / LPR;
+    const int q = lane % LPR;
+    const int mode = e.mode;
+    const int ndot = dev_ndot(mode);
+
+    auto ld = [](const S *p, S (&v)[NS]) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const C16 t = *reinterpret_cast<const C16 *>(p + c * EPC);
+#pragma unroll
+            for (int k = 0; k < EPC; ++k) v[c * EPC + k] = t[k];
+        }
+    };
+    auto vt = [](const S (&v)[NS], int j) -> VT {
+        if constexpr (CV) return VT{v[2 * j], v[2 * j + 1]};
+        else return v[j];
+    };
+
+    double2 dacc[3][CPL];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) dacc[d][j] = double2{0.0, 0.0};
+
+    const int64_t rows_per_wg = (int64_t)kWaves * G;
+    for (int64_t base = (int64_t)blockIdx.x * rows_per_wg; base < nbr; base += (int64_t)gridDim.x * rows_per_wg) {
+        const int64_t br = base + (int64_t)wave * G + g;
+        if (br >= nbr) continue;
+        S acc[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) acc[i] = S(0);
+        const int64_t s0 = ulen > 0 ? br * ulen : row_ptr[br];
+        const int64_t s1 = ulen > 0 ? s0 + ulen : row_ptr[br + 1];
+        const int64_t lo = (int64_t)q * NS;
+        int64_t s = s0;
+        constexpr int U = 4;
+        for (; s + U <= s1; s += U) {
+            int64_t bc[U];
+            S a[U];
+#pragma unroll
+            for (int t = 0; t < U; ++t) {
+                bc[t] = col_ind[s + t];
+                a[t] = vals[s + t];
+            }
+            S xk[U][NS];
+#pragma unroll
+            for (int t = 0; t < U; ++t) ld(X + bc[t] * W + lo, xk[t]);
+#pragma unroll
+            for (int t = 0; t < U; ++t)
+#pragma unroll
+                for (int i = 0; i < NS; ++i) mac(acc[i], a[t], xk[t][i]);
+        }
+        for (; s < s1; ++s) {
+            const int64_t bc = col_ind[s];
+            const S a = vals[s];
+            S xk[NS];
+            ld(X + bc * W + lo, xk);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) mac(acc[i], a, xk[i]);
+        }
+
+        const int64_t idx = br * W + lo;
+        S y[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) y[i] = acc[i];
+        if (mode == EPI_FIRST) {
+            S xs[NS];
+            ld(XS + idx, xs);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = affine(y[i], e.alpha, e.beta, xs[i]);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                dotacc(dacc[0][j], vt(xs, j), vt(xs, j));
+                dotacc(dacc[1][j], vt(xs, j), vt(y, j));
+                dotacc(dacc[2][j], vt(y, j), vt(y, j));
+            }
+        } else if (mode == EPI_STEP) {
+            S xs[NS], xp[NS];
+            ld(XS + idx, xs);
+            ld(XP + idx, xp);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = sub(affine(y[i], e.alpha, e.beta, xs[i]), xp[i]);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                dotacc(dacc[0][j], vt(y, j), vt(y, j));
+                dotacc(dacc[1][j], vt(y, j), vt(xs, j));
+            }
+        } else if (mode == EPI_PLAINSTEP) {
+            S xs[NS], xp[NS], ps[NS];
+            ld(XS + idx, xs);
+            ld(XP + idx, xp);
+            ld(PSI + idx, ps);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = sub(affine(y[i], e.alpha, e.beta, xs[i]), xp[i]);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) dotacc(dacc[0][j], vt(ps, j), vt(y, j));
+        } else if (mode == EPI_ADJ) {
+            S xs[NS], xp[NS], t[NS];
+            ld(XS + idx, xs);
+            ld(XP + idx, xp);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = sub(affine(y[i], e.alpha, e.beta, xs[i]), xp[i]);
+            ld(static_cast<const S *>(e.qm) + idx, t);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = affine(y[i], 1.0, e.cm, t[i]);
+            ld(static_cast<const S *>(e.qz) + idx, t);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = affine(y[i], 1.0, e.cz, t[i]);
+            ld(static_cast<const S *>(e.qp) + idx, t);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) y[i] = affine(y[i], 1.0, e.cp, t[i]);
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            C16 t;
+#pragma unroll
+            for (int k = 0; k < EPC; ++k) t[k] = y[c * EPC + k];
+            *reinterpret_cast<C16 *>(Y + idx + c * EPC) = t;
+        }
+    }
+
+    if (ndot == 0) return;
+    // the rows of one wave hold the same columns: fold them (xor tree)
+    if (G > 1) {
+#pragma unroll
+        for (int off = LPR; off < 64; off <<= 1)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) {
+                    dacc[d][j].x += __shfl_xor(dacc[d][j].x, off);
+                    dacc[d][j].y += __shfl_xor(dacc[d][j].y, off);
+                }
+    }
+    constexpr int COLS = LPR * CPL;
+    __shared__ double2 sm[kWaves][COLS];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= ndot) break;
+        if (g == 0) {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) sm[wave][q * CPL + j] = dacc[d][j];
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < COLS; u += blockDim.x) {
+            double2 acc2 = sm[0][u];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) {
+                acc2.x += sm[w][u].x;
+                acc2.y += sm[w][u].y;
+            }
+            e.partials[((int64_t)blockIdx.x * ndot + d) * COLS + u] = acc2;
+        }
+        __syncthreads();
+    }
+}
+
+// Lanes per row of the slab layout for this launch shape, or 0 when it does
+// not apply: b = 1, real matrix, R of 16 or 32, g_b1v_bytes a lane.
+int b1v_lpr(const BsrDev &A, bool fp32, bool cv, int R) {
+    if (A.b != 1 || A.is_complex || g_b1v_bytes == 0 || (R != 16 && R != 32)) return 0;
+    const int vt = (fp32 ? 4 : 8) * (cv ? 2 : 1);
+    const int cpl = g_b1v_bytes / vt;
+    if (cpl < 1 || R % cpl != 0) return 0;
+    const int lpr = R / cpl;
+    const int64_t rows_per_wg = (int64_t)kWaves * (64 / lpr);
+    if ((A.nbr + rows_per_wg - 1) / rows_per_wg < g_b1v_min_grid) return 0;
+    return lpr;
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+template <typename S, bool CV, int LPR, int CPL>
+hipError_t go_b1v(const BsrDev &A, const void *X, void *Y, const EpiArgs &e, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_spmm_b1v<S, CV, LPR, CPL>), dim3(grid), dim3(64 * kWaves), 0, st, A.nbr,
+                       g_uniform_rows ? A.ulen : 0, A.row_ptr, A.col_ind, static_cast<const S *>(A.vals),
+                       static_cast<const S *>(X), static_cast<S *>(Y), e);
+    return hipGetLastError();
+}
+
+// The slab launch for lpr = b1v_lpr(...) (nonzero).
+template <typename S, bool CV>
+hipError_t by_b1v(int lpr, const BsrDev &A, const void *X, void *Y, int R, const EpiArgs &e, int grid,
+                  hipStream_t st) {
+    constexpr int VT = (int)sizeof(S) * (CV ? 2 : 1);
+    auto pick = [&](auto bytes_c) -> hipError_t {
+        constexpr int CPL = decltype(bytes_c)::value / VT;
+        if constexpr (CPL < 1 || 16 % CPL != 0 || (CPL * VT) % 16 != 0) {
+            return hipErrorInvalidValue;
+        } else {
+            if (R == 16) return go_b1v<S, CV, 16 / CPL, CPL>(A, X, Y, e, grid, st);
+            if constexpr (32 / CPL <= 64) return go_b1v<S, CV, 32 / CPL, CPL>(A, X, Y, e, grid, st);
+            return hipErrorInvalidValue;
+        }
+    };
+    const int bytes = R / lpr * VT;
+    if (bytes == 16) return pick(std::integral_constant<int, 16>{});
+    if (bytes == 32) return pick(std::integral_constant<int, 32>{});
+    if (bytes == 64) return pick(std::integral_constant<int, 64>{});
+    return hipErrorInvalidValue;
 }
 
 
@@ -939,23 +1183,25 @@ __global__ __launch_bounds__(256) void k_reduce(const double2 *__restrict__ part
     partials += (int64_t)blockIdx.y * pstride;
     out += (int64_t)blockIdx.y * ostride;
     const int lane = threadIdx.x & 63;
-    const int pair = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const int pair = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     const int npairs = ndot * R;
     if (pair >= npairs) return;
     const int d = pair / R;
     const int u = pair - d * R;
     // compensated FP64 sums over the workgroup partials, then a fixed xor tree
     double sx = 0.0, cx = 0.0, sy = 0.0, cy = 0.0;
-    // eight loads in flight per lane before they are summed in order
-    for (int g0 = lane; g0 < ngroups; g0 += 64 * 8) {
-        double2 p[8];
+    // 32 loads in flight per lane (all of a 2048-workgroup launch's partials
+    // in one round trip) before they are summed in order
+    constexpr int B = 32;
+    for (int g0 = lane; g0 < ngroups; g0 += 64 * B) {
+        double2 p[B];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < B; ++q) {
             const int gi = g0 + 64 * q;
             p[q] = gi < ngroups ? partials[((int64_t)gi * ndot + d) * R + u] : double2{0.0, 0.0};
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < B; ++q) {
             two_sum_acc(sx, cx, p[q].x);
             two_sum_acc(sy, cy, p[q].y);
         }
@@ -1142,6 +1388,17 @@ hipError_t launch_spmm(const BsrDev &A, VecMode vm, const void *X, void *Y, int 
     // the rows per workgroup)
     const int grid = valu_grid(A.nbr, valu_lpr(A, vm.fp32, vm.complex_vec, R));
     if (grid_out) *grid_out = grid;
+    if (const int lpr = b1v_lpr(A, vm.fp32, vm.complex_vec, R)) {
+        const void *ops[] = {X, Y, e.xs, e.xp, e.psi, e.qm, e.qz, e.qp};
+        bool ok = true;
+        for (const void *o : ops) ok = ok && aligned16(o);
+        if (ok) {
+            if (vm.fp32) return vm.complex_vec ? by_b1v<float, true>(lpr, A, X, Y, R, e, grid, stream)
+                                               : by_b1v<float, false>(lpr, A, X, Y, R, e, grid, stream);
+            return vm.complex_vec ? by_b1v<double, true>(lpr, A, X, Y, R, e, grid, stream)
+                                  : by_b1v<double, false>(lpr, A, X, Y, R, e, grid, stream);
+        }
+    }
     if (!vm.fp32) {
         if (A.is_complex) return by_block<double, true, true>(A, X, Y, R, e, grid, stream);
         if (vm.complex_vec) return by_block<double, false, true>(A, X, Y, R, e, grid, stream);
@@ -1159,6 +1416,8 @@ void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_mfma_grid_cap(int g) { g_mfma_grid_cap = g >= 8 && g <= kMaxGrid ? (g / 8) * 8 : 0; }
 void set_b1_wide(int mode) { g_b1_wide = mode == 0 || mode == 1 ? mode : 2; }
 void set_uniform_rows(int on) { g_uniform_rows = on != 0; }
+void set_b1_slab(int bytes) { g_b1v_bytes = bytes == 0 || bytes == 16 || bytes == 32 || bytes == 64 ? bytes : 16; }
+void set_b1_slab_min_grid(int g) { g_b1v_min_grid = g >= 0 ? g : 512; }
 void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
 void set_mfma_pd16(int fp32, int step, int p) {
     if (p >= 2 && p <= 6) g_pd16[fp32 ? 1 : 0][step ? 1 : 0] = p;
@@ -1180,10 +1439,11 @@ hipError_t launch_reduce(const double2 *partials, int ngroups, int ndot, int R, 
 hipError_t launch_reduce_steps(const double2 *partials, int64_t pstride, int ngroups, int ndot, int R,
                                double2 *out, int64_t ostride, int nsteps, hipStream_t stream) {
     if (nsteps < 1) return hipSuccess;
+    // one wave (one dot and column) a workgroup: the strided partial loads
+    // spread over as many compute units as there are sums
     const int npairs = ndot * R;
-    const int grid = (npairs + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(k_reduce, dim3(grid, (unsigned)nsteps), dim3(64 * kWaves), 0, stream, partials, ngroups,
-                       ndot, R, out, pstride, ostride);
+    hipLaunchKernelGGL(k_reduce, dim3(npairs, (unsigned)nsteps), dim3(64), 0, stream, partials, ngroups, ndot, R,
+                       out, pstride, ostride);
     return hipGetLastError();
 }
 
