@@ -17,7 +17,9 @@ namespace {
 // registers while the group walks the row's entries, gathering each V (V2)
 // row with coalesced loads and summing the lane partials by xor shuffles.
 // Entries are owned by their point row, so no two groups write the same G
-// element. For b = 1, four entries' rows are loaded before their reductions.
+// element. For b = 1, four entries' rows are loaded before their reductions,
+// and their four G values with the column indices (every lane of the group
+// loads them; lane 0 writes the sums back), so no load waits on a reduction.
 template <bool CPLX, int LPR, int UPL, bool B1, bool PAIR2>
 __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const int64_t *__restrict__ row_ptr,
                                                     const int32_t *__restrict__ col_ind,
@@ -63,10 +65,16 @@ __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const in
         if constexpr (B1) {
             constexpr int K = 4;
             for (; s + K <= s1; s += K) {
-                double acc[K];
+                double acc[K], g0[K];
+                int64_t jk[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const int64_t j = col_ind[s + k];
+                    jk[k] = col_ind[s + k];
+                    g0[k] = G[s + k];
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int64_t j = jk[k];
                     acc[k] = 0.0;
 #pragma unroll
                     for (int t = 0; t < UPL; ++t) {
@@ -80,7 +88,7 @@ __global__ __launch_bounds__(256) void k_grad_accum(int64_t nbr, int b, const in
                 for (int k = 0; k < K; ++k) acc[k] = reduce(acc[k]);
                 if (q == 0)
 #pragma unroll
-                    for (int k = 0; k < K; ++k) G[s + k] += coef * acc[k];
+                    for (int k = 0; k < K; ++k) G[s + k] = g0[k] + coef * acc[k];
             }
         }
         for (; s < s1; ++s) {
@@ -162,11 +170,21 @@ hipError_t go_grad(int64_t nbr, int b, const int64_t *row_ptr, const int32_t *co
     return hipGetLastError();
 }
 
+// Sample columns a lane for 8 < R <= 32: 1 (one column, LPR >= R lanes a
+// row), 2 (half the lanes a row, twice the rows a wave) or 4 (R <= 32: 8 lanes
+// a row). Measured on the C4-shaped adjoint (65,536 rows x 13 entries, R = 32
+// complex): backward 65.4 / 65.8 ms with one column a lane, 61.8 / 60.9 ms
+// with two.
+int g_grad_wide = 2;
+
 template <bool CPLX>
 hipError_t grad_by_r(int64_t nbr, int b, const int64_t *row_ptr, const int32_t *col_ind, const double *U,
                      const double *V, const double *U2, const double *V2, int R, double coef, double *G,
                      hipStream_t st) {
     if (R <= 8) return go_grad<CPLX, 8, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (g_grad_wide >= 2 && R <= 16) return go_grad<CPLX, 8, 2>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (g_grad_wide == 2 && R <= 32) return go_grad<CPLX, 16, 2>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
+    if (g_grad_wide == 4 && R <= 32) return go_grad<CPLX, 8, 4>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
     if (R <= 16) return go_grad<CPLX, 16, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
     if (R <= 32) return go_grad<CPLX, 32, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
     if (R <= 64) return go_grad<CPLX, 64, 1>(nbr, b, row_ptr, col_ind, U, V, U2, V2, R, coef, G, st);
@@ -204,5 +222,7 @@ hipError_t launch_axpby(double *y, const double *x, double alpha, double beta, i
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(256), 0, st, y, x, alpha, beta, n);
     return hipGetLastError();
 }
+
+void set_grad_wide(int cols) { g_grad_wide = cols == 1 || cols == 4 ? cols : 2; }
 
 } // namespace rg

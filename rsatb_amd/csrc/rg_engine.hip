@@ -1232,6 +1232,7 @@ int rg_create(int device_id, rg_context **out) {
     if (const char *v = std::getenv("RG_ULEN")) set_uniform_rows(std::atoi(v));
     if (const char *v = std::getenv("RG_B1V")) set_b1_slab(std::atoi(v));
     if (const char *v = std::getenv("RG_B1V_MIN")) set_b1_slab_min_grid(std::atoi(v));
+    if (const char *v = std::getenv("RG_GRAD_WIDE")) set_grad_wide(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PLAN")) ctx->mfma_plan = std::atoi(v) != 0;
     if (const char *v = std::getenv("RG_MFMA_PD16")) {
         // "fp64 plain, fp64 step, fp32 plain, fp32 step"

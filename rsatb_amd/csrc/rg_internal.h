@@ -88,6 +88,9 @@ void set_b1_slab(int bytes);
 // fewest workgroups of a slab launch (smaller problems take the other
 // layouts); negative: the default
 void set_b1_slab_min_grid(int g);
+// sample columns a lane (1, 2 or 4; anything else: 2) in the gradient
+// accumulation for 8 < R <= 32
+void set_grad_wide(int cols);
 void set_mfma_pd32(int p);
 void set_mfma_pd64(int p);
 // prefetch depth (2 .. 6) of the sixteen-tile complex MFMA SpMM for one

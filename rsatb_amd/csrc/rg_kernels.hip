@@ -62,7 +62,10 @@ int g_uniform_rows = 1;
 // the 16,384-row graphene run (FP64 / FP32) 2.90 / 2.73 ms with k_spmm,
 // 2.61 / 2.86 ms with 16-byte slabs: the FP32 slab launch there has 256
 // workgroups, one wave per SIMD.
-int g_b1v_bytes = 16;
+// Automatic (-1): 16 bytes, 32 for complex FP64 vectors (two columns a lane:
+// the C4-shaped adjoint, 65,536 rows x 13 entries, R = 32 complex, forward
+// 25.1 -> 23.7 ms and backward 70.4 -> 69.5 ms against 16 bytes).
+int g_b1v_bytes = -1;
 int g_b1v_min_grid = 512;
 // Register prefetch depths of the MFMA SpMM: FP64 plain products 3 (2 when
 // set to 2; steps always 2: their epilogue needs the registers), FP32 3 or 4.
@@ -591,7 +594,8 @@ __global__ __launch_bounds__(256) void k_spmm_b1v(int64_t nbr, int ulen, const i
 int b1v_lpr(const BsrDev &A, bool fp32, bool cv, int R) {
     if (A.b != 1 || A.is_complex || g_b1v_bytes == 0 || (R != 16 && R != 32)) return 0;
     const int vt = (fp32 ? 4 : 8) * (cv ? 2 : 1);
-    const int cpl = g_b1v_bytes / vt;
+    const int bytes = g_b1v_bytes > 0 ? g_b1v_bytes : (vt == 16 ? 32 : 16);
+    const int cpl = bytes / vt;
     if (cpl < 1 || R % cpl != 0) return 0;
     const int lpr = R / cpl;
     const int64_t rows_per_wg = (int64_t)kWaves * (64 / lpr);
@@ -1416,7 +1420,7 @@ void set_mfma_ntcap(int nt) { g_mfma_ntcap = nt; }
 void set_mfma_grid_cap(int g) { g_mfma_grid_cap = g >= 8 && g <= kMaxGrid ? (g / 8) * 8 : 0; }
 void set_b1_wide(int mode) { g_b1_wide = mode == 0 || mode == 1 ? mode : 2; }
 void set_uniform_rows(int on) { g_uniform_rows = on != 0; }
-void set_b1_slab(int bytes) { g_b1v_bytes = bytes == 0 || bytes == 16 || bytes == 32 || bytes == 64 ? bytes : 16; }
+void set_b1_slab(int bytes) { g_b1v_bytes = bytes == 0 || bytes == 16 || bytes == 32 || bytes == 64 ? bytes : -1; }
 void set_b1_slab_min_grid(int g) { g_b1v_min_grid = g >= 0 ? g : 512; }
 void set_mfma_pd32(int p) { g_mfma_pd32 = p == 4 ? 4 : 3; }
 void set_mfma_pd16(int fp32, int step, int p) {

@@ -108,3 +108,30 @@ def test_slab_adjoint_agrees(slab_env):
             eng.ctx.close()
     for v in SLABS[1:]:
         assert rel(grads[v], grads["0"]) <= 1e-12
+
+
+def test_gradient_accumulation_layouts_agree(monkeypatch):
+    # one, two or four sample columns a lane in the pattern contraction
+    # dL/dA_ij = sum_c Re(u_i conj v_j): the same sums in another order
+    st = graphene_rect_supercell(8, 8)
+    model = sk_pz_parameterization(st, neighbor_edges(st, 1.6), [1.6])
+    theta = np.array([0.0, -2.7])
+    g = np.linspace(1.0, -1.0, 90)
+    grads = {}
+    try:
+        for R in (16, 32):
+            for cols in ("1", "2", "4"):
+                monkeypatch.setenv("RG_GRAD_WIDE", cols)
+                _lib.Context(0).close()  # the knob is read at context creation (process-wide)
+                eng = DifferentiableKPM(model, R, (-8.2, 8.2), 90, checkpoint_stride=8)
+                try:
+                    eng.forward(theta)
+                    grads[R, cols] = eng.ctx.kpm_moments_backward(g)
+                finally:
+                    eng.ctx.close()
+    finally:
+        monkeypatch.setenv("RG_GRAD_WIDE", "-1")  # the default
+        _lib.Context(0).close()
+    for R in (16, 32):
+        for cols in ("2", "4"):
+            assert rel(grads[R, cols], grads[R, "1"]) <= 1e-12
