@@ -76,9 +76,12 @@ typedef struct rg_context rg_context;
 #define RG_SLOT_DVZ 10
 #define RG_SLOT_COUNT 11
 
-/* value types of an uploaded matrix */
+/* value types of an uploaded matrix (FP32 / complex-FP32 values are widened
+ * to FP64 exactly on upload) */
 #define RG_F64 0
+#define RG_F32 1
 #define RG_C128 2
+#define RG_C64 3
 
 /* operator forms (ChebyOpForm, cheby.hpp:78-82; Solve is not provided) */
 #define RG_FORM_ORTHOGONAL 0
@@ -123,8 +126,9 @@ RG_API int rg_version(void);
 
 /* Replaces BlockSpMat<T> assembly for one operator slot (H, Z, Z^H, S):
  * rsatb/setup/operator_set.hpp:291-298 produces H(k)/S(k), and
- * preconditioner.hpp:184-196 owns Z. `values` holds nnzb*b*b doubles (F64)
- * or nnzb*b*b (re, im) pairs (C128). */
+ * preconditioner.hpp:184-196 owns Z. `values` holds nnzb*b*b doubles (F64),
+ * floats (F32), or nnzb*b*b (re, im) pairs of doubles (C128) or floats (C64),
+ * each block row-major. */
 RG_API int rg_upload_bsr(rg_context *ctx, int slot, int64_t n_block_rows, int block_dim,
                   const int64_t *row_ptr, const int32_t *col_ind, const void *values,
                   int dtype);

@@ -24,7 +24,7 @@ RG_ERR_UNSUPPORTED = 6
 SLOT_H, SLOT_Z, SLOT_ZH, SLOT_S = 0, 1, 2, 3
 SLOT_VX, SLOT_VY, SLOT_VZ = 4, 5, 6
 SLOT_DH, SLOT_DVX, SLOT_DVY, SLOT_DVZ = 7, 8, 9, 10
-F64, C128 = 0, 2
+F64, F32, C128, C64 = 0, 1, 2, 3
 FORM_ORTHOGONAL, FORM_SANDWICH = 0, 1
 SAMPLES_RADEMACHER, SAMPLES_PHASE, SAMPLES_UPLOAD_REAL, SAMPLES_UPLOAD_COMPLEX = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
@@ -212,7 +212,11 @@ class Context:
     # operands -------------------------------------------------------
     def upload_bsr(self, slot: int, bsr) -> None:
         vals = np.ascontiguousarray(bsr.values)
-        if np.iscomplexobj(vals):
+        if vals.dtype == np.float32:
+            dtype = F32  # widened to FP64 by the library
+        elif vals.dtype == np.complex64:
+            dtype = C64
+        elif np.iscomplexobj(vals):
             vals = vals.astype(np.complex128, copy=False)
             dtype = C128
         else:

@@ -18,7 +18,7 @@ class BSR:
     block_dim: int
     row_ptr: np.ndarray  # int64 [nbr + 1]
     col_ind: np.ndarray  # int32 [nnzb]
-    values: np.ndarray   # [nnzb, b, b] float64 or complex128
+    values: np.ndarray   # [nnzb, b, b] float64 or complex128 (float32 / complex64 widened on upload)
 
     @property
     def nnzb(self) -> int:

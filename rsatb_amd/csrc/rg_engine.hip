@@ -1289,8 +1289,8 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     if (slot < 0 || slot >= RG_SLOT_COUNT) return fail(ctx, RG_ERR_INVALID, "unknown matrix slot");
     if (nbr < 1 || b < 1 || b > 16) return fail(ctx, RG_ERR_INVALID, "block rows >= 1 and 1 <= b <= 16");
     if (!row_ptr || !col_ind || !values) return fail(ctx, RG_ERR_INVALID, "null array");
-    if (dtype != RG_F64 && dtype != RG_C128)
-        return fail(ctx, RG_ERR_UNSUPPORTED, "values must be RG_F64 or RG_C128");
+    if (dtype != RG_F64 && dtype != RG_C128 && dtype != RG_F32 && dtype != RG_C64)
+        return fail(ctx, RG_ERR_UNSUPPORTED, "values must be RG_F64, RG_F32, RG_C128 or RG_C64");
     if (row_ptr[0] != 0) return fail(ctx, RG_ERR_INVALID, "row_ptr[0] must be 0");
     for (int64_t i = 0; i < nbr; ++i)
         if (row_ptr[i + 1] < row_ptr[i]) return fail(ctx, RG_ERR_INVALID, "row_ptr must be non-decreasing");
@@ -1303,7 +1303,15 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     s.present = false;
     s.vals32_valid = false;
     touch_slot(ctx, s);
-    const size_t nv = (size_t)nnzb * b * b * (dtype == RG_C128 ? 2 : 1);
+    const bool cplx = dtype == RG_C128 || dtype == RG_C64;
+    const size_t nv = (size_t)nnzb * b * b * (cplx ? 2 : 1);
+    // FP32 values widened to FP64 on the host (exact)
+    std::vector<double> wide;
+    if (dtype == RG_F32 || dtype == RG_C64) {
+        const float *f = static_cast<const float *>(values);
+        wide.assign(f, f + nv);
+        values = wide.data();
+    }
     RG_HIP(ctx, s.row_ptr.ensure((size_t)(nbr + 1) * 8), "hipMalloc row_ptr");
     RG_HIP(ctx, s.col_ind.ensure((size_t)nnzb * 4 + 4), "hipMalloc col_ind");
     RG_HIP(ctx, s.vals64.ensure(nv * 8), "hipMalloc values");
@@ -1317,7 +1325,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
         if (he == hipSuccess)
             he = launch_block_transpose(static_cast<const double *>(staging.p),
                                         static_cast<double *>(s.vals64.p), nnzb, b,
-                                        dtype == RG_C128, ctx->stream);
+                                        cplx, ctx->stream);
         if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
         staging.release();
         if (he != hipSuccess) return hipfail(ctx, he, "upload values");
@@ -1325,7 +1333,7 @@ int rg_upload_bsr(rg_context *ctx, int slot, int64_t nbr, int b, const int64_t *
     s.nbr = nbr;
     s.b = b;
     s.nnzb = nnzb;
-    s.is_complex = (dtype == RG_C128);
+    s.is_complex = cplx;
     int rcg = build_xrow(ctx, s, row_ptr, col_ind);
     if (rcg) return rcg;
     s.present = true;

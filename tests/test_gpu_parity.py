@@ -312,3 +312,22 @@ def test_uniform_rows(monkeypatch, oracle, b, L, R):
     assert rel_err(got["1"][0], oracle.bsr_mult(A, X)) < FP64_TOL
     assert rel_err(got["1"][1], oracle.bsr_mult(A2, X)) < FP64_TOL
     assert np.array_equal(got["1"][0], got["0"][0]) and np.array_equal(got["1"][1], got["0"][1])
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_fp32_value_upload(gpu_ctx, cplx):
+    # F32 / C64 values are widened to FP64 exactly: the stored slot and the
+    # products equal those of the widened values uploaded as F64 / C128
+    rng = np.random.default_rng(41 + cplx)
+    A = random_bsr(40, 3, rng, complex_=cplx)
+    lo = A.values.astype(np.complex64 if cplx else np.float32)
+    A32 = BSR(A.n_block_rows, A.block_dim, A.row_ptr, A.col_ind, lo)
+    A64 = BSR(A.n_block_rows, A.block_dim, A.row_ptr, A.col_ind, lo.astype(np.complex128 if cplx else np.float64))
+    X = rng.standard_normal((A.n_rows, 8)) + 1j * rng.standard_normal((A.n_rows, 8))
+    gpu_ctx.upload_bsr(_lib.SLOT_H, A32)
+    got = gpu_ctx.download_bsr(_lib.SLOT_H)
+    y32 = gpu_ctx.bsr_mult(_lib.SLOT_H, X)
+    gpu_ctx.upload_bsr(_lib.SLOT_H, A64)
+    y64 = gpu_ctx.bsr_mult(_lib.SLOT_H, X)
+    assert np.array_equal(got.values, A64.values) and np.array_equal(y32, y64)
+    assert gpu_ctx.bsr_info(_lib.SLOT_H)["is_complex"] == cplx
