@@ -211,6 +211,8 @@ struct rg_context {
     // MFMA SpMM plans (row schedule, colour-ordered blocks) for real b >= 8
     // slots written from now on (RG_MFMA_PLAN=0 turns them off)
     bool mfma_plan = true;
+    // rows a plan region holds (0: plan_region_rows; RG_MFMA_SREG for sweeps)
+    int32_t mfma_sreg = 0;
     // set while rg_kpm_moments_noalloc runs: nothing may allocate
     bool noalloc_active = false;
     std::vector<hipGraphExec_t> graphs;
@@ -501,7 +503,8 @@ int build_xrow(rg_context *ctx, Slot &s, const int64_t *rp, const int32_t *ci) {
     RG_HIP(ctx, hipMemcpy(s.xrow.p, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload gather table");
     s.has_xrow = true;
     if (!ctx->mfma_plan || s.nnzb >= INT32_MAX || s.nbr >= INT32_MAX) return RG_OK;
-    MfmaPlan pl = mfma_plan(s.nbr, rp, ci, plan_region_rows(s.nbr));
+    const int32_t sreg = ctx->mfma_sreg > 0 ? (int32_t)std::min<int64_t>(ctx->mfma_sreg, s.nbr) : plan_region_rows(s.nbr);
+    MfmaPlan pl = mfma_plan(s.nbr, rp, ci, sreg);
     std::vector<int32_t> kc((size_t)s.nnzb + 1, 0);
     for (int64_t q = 0; q < s.nnzb; ++q) {
         kc[(size_t)q] = ci[pl.kperm[(size_t)q]];
@@ -1234,6 +1237,7 @@ int rg_create(int device_id, rg_context **out) {
     if (const char *v = std::getenv("RG_B1V_MIN")) set_b1_slab_min_grid(std::atoi(v));
     if (const char *v = std::getenv("RG_GRAD_WIDE")) set_grad_wide(std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PLAN")) ctx->mfma_plan = std::atoi(v) != 0;
+    if (const char *v = std::getenv("RG_MFMA_SREG")) ctx->mfma_sreg = std::max(0, std::atoi(v));
     if (const char *v = std::getenv("RG_MFMA_PD16")) {
         // "fp64 plain, fp64 step, fp32 plain, fp32 step"
         int p[4] = {0, 0, 0, 0};
