@@ -1,5 +1,6 @@
 """The b = 1 slab SpMM (a lane owns 16, 32 or 64 contiguous bytes of a row)
-against the column-per-lane layout, on uniform (graphene) and ragged rows,
+against the column-per-lane layout, on uniform rows (graphene: 4 entries; bands
+of 9 and 13 entries) and ragged rows,
 R = 16 and 32, real and complex vectors, FP64 and FP32: products bit for bit
 (the same operations in the same order), moments and adjoint gradients to
 1e-12 (only the dots' summation order differs), and FP64 moments against the
@@ -42,11 +43,23 @@ def ragged(rng, nbr=700):
     return BSR.from_blocks(nbr, 1, r, c, D[r, c].reshape(-1, 1, 1))
 
 
+def band(rng, m, nbr=700):
+    # symmetric circulant band: every row holds exactly 2 m + 1 entries
+    D = np.zeros((nbr, nbr))
+    for k in range(1, m + 1):
+        v = rng.standard_normal(nbr)
+        for i in range(nbr):
+            D[i, (i + k) % nbr] = D[(i + k) % nbr, i] = v[i]
+    D[np.arange(nbr), np.arange(nbr)] = rng.standard_normal(nbr)
+    r, c = np.nonzero(D)
+    return BSR.from_blocks(nbr, 1, r, c, D[r, c].reshape(-1, 1, 1))
+
+
 def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("shape", ["graphene", "ragged"])
+@pytest.mark.parametrize("shape", ["graphene", "ragged", "band9", "band13"])
 @pytest.mark.parametrize("R", [16, 32])
 @pytest.mark.parametrize("samples", ["rademacher", "phase"])
 def test_slab_layouts_agree(slab_env, oracle, shape, R, samples):
@@ -56,7 +69,7 @@ def test_slab_layouts_agree(slab_env, oracle, shape, R, samples):
         H = assemble(st, neighbor_edges(st, 1.6), SkPzModel())
         half = 8.1 / 0.99
     else:
-        H = ragged(rng)
+        H = ragged(rng) if shape == "ragged" else band(rng, 4 if shape == "band9" else 6)
         half = 1.01 * np.max(np.abs(np.linalg.eigvalsh(H.to_dense())))
     kind = _lib.SAMPLES_RADEMACHER if samples == "rademacher" else _lib.SAMPLES_PHASE
     X = rng.standard_normal((H.n_rows, R))
